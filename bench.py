@@ -1,0 +1,186 @@
+"""Benchmark of the GeodesicFDTD step loop on one MI355X.
+
+Workload (default): s7 icosahedral mesh (163842 dual cells) x 64 radial cells
+to 100 km, FP32 fields, exponential conductive update with
+sigma(z) = 1e-6 exp((z-70 km)/5 km) S/m, surface-impedance ground
+(sigma_g = 1e-2 S/m), electron Drude plasma above 60 km, 20 Hz Gaussian
+vertical current, 32 receivers sampled every step.  A "step" is one full
+leapfrog step (H kernel + E kernel) of this grid; inputs are resident in HBM.
+
+Prints one JSON line: Mcell-updates/s (steps/s x V x R), the roofline of the
+dominant kernel (algorithmic bytes / measured kernel time vs 8 TB/s) and the
+CPU oracle (the NumPy restatement of the reference path) timed on a bounded
+sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+REPO = Path(__file__).resolve().parent
+sys.path.insert(0, str(REPO))
+sys.path.insert(0, str(REPO / "tests"))
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcell-updates/s + % HBM roofline (1 GPU); ensemble trace queries/s at ≤2% err"
+
+
+def algorithmic_bytes(sim) -> dict:
+    """Compulsory bytes per launch of each kernel.
+
+    Fields: every field value is read once by the half-step that uses it as a
+    source and read + written once by the half-step that updates it
+    (3 accesses per DOF and step).  Plasma current: read + written by the H
+    kernel (3 components per species and band cell).  Surface impedance:
+    memory read + written.  Tables: index / metric rows read once per entity.
+    """
+    V, E, F = sim.mesh.n_vertices, sim.mesh.n_edges, sim.mesh.n_faces
+    R = len(sim.radial_midpoints_m)
+    sf = sim._er_buf.element_size()
+    sc = 8 if sim.backend.dtype_name == "float64" else 4
+    lay = sim._layout
+    band = (lay.band_hi - lay.band_lo) if sim._plasma_buf is not None else 0
+    species = sim._plasma_buf.shape[0] if sim._plasma_buf is not None else 0
+    si_terms = sim._si_memory.shape[1] if sim._si_memory is not None else 0
+    h_fields = sf * (V * (R + 1) + E * R + 2 * E * (R + 1) + 2 * F * R)
+    h_plasma = sc * 6 * F * band * species
+    h_si = sc * 2 * si_terms * E
+    h_geom = 12 * E + 24 * F + (60 * F if species else 0)
+    e_fields = sf * (E * (R + 1) + F * R + 2 * V * (R + 1) + 2 * E * R)
+    e_geom = 48 * V + 12 * E + ((48 * V + 24 * E + 12 * F) if species else 0)
+    return {"h": h_fields + h_plasma + h_si + h_geom, "e": e_fields + e_geom,
+            "h_fields": h_fields, "h_plasma": h_plasma, "h_si": h_si, "h_geom": h_geom,
+            "e_fields": e_fields, "e_geom": e_geom}
+
+
+def read_traffic(kernel: str) -> float | None:
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
+    scaled by the copy-kernel calibration), from the committed summary."""
+    path = REPO / "profiles" / f"{kernel}_c1_traffic.txt"
+    if not path.exists():
+        return None
+    for line in path.read_text().splitlines():
+        if line.startswith("bytes_per_launch="):
+            return float(line.split("=", 1)[1])
+    return None
+
+
+def cpu_baseline(steps: int = 3) -> dict:
+    """The CPU oracle (NumPy restatement of the reference array algorithm,
+    FP32 fields) on the same C1 grid, a few steps after one warm-up step."""
+    import numpy as np
+    import scenarios as S
+
+    import ionosphere_fdtd_amd as g
+
+    o = S.oracle_module()
+    sc = S.scenario("c1")
+    mesh = g.build_geodesic_mesh(7)  # bit-identical to the oracle's own builder
+    alt = S._midpoint_altitudes(sc)
+    b, species = S._plasma_arrays(sc, mesh, alt)
+    plasma = o.OraclePlasma(b, [o.OracleSpecies(q, m, n, nu) for _, q, m, n, nu in species])
+    sim = o.OracleSim(subdivision=7, radial_cells=64, minimum_altitude_m=0.0,
+                      maximum_altitude_m=100e3, earth_radius_m=sc["earth_radius_m"],
+                      material=sc["material"], surface=o.OracleSurface(sc["surface_sigma"]),
+                      plasma=plasma, mesh=mesh, dtype="float32",
+                      source=o.OracleSource(latitude_deg=S.SOURCE_LAT,
+                                            longitude_deg=S.SOURCE_LON, altitude_m=0.0,
+                                            carrier_frequency_hz=20.0))
+    sim.step(1)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    elapsed = time.perf_counter() - t0
+    cells = mesh.n_vertices * 64
+    return {"value": steps * cells / elapsed / 1e6, "unit": "Mcell-updates/s",
+            "cores": 1, "kind": "port",
+            "sample": f"{steps} FP32 steps of the s7 x 64 plasma+SI workload after 1 warm-up "
+                      f"step ({elapsed:.1f} s; NumPy ufuncs single-threaded; host has "
+                      f"{os.cpu_count()} logical CPUs)"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--chunk", type=int, default=100)
+    ap.add_argument("--kernel-steps", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the multi-GPU halo path is out of scope for this "
+              "build; run with --gpus 1", file=sys.stderr)
+        return 2
+    import numpy as np
+    import torch
+
+    import scenarios as S
+
+    torch.cuda.set_device(0)
+    sc = S.scenario("c1")
+    t_setup = time.perf_counter()
+    sim = S.build_sim(sc, compile_step=True, compile_chunk_size=args.chunk)
+    pts = S.receiver_points(sc)
+    v, l, w = S.receiver_tables(sim.mesh, sim.altitudes_m, pts)
+    setup_s = time.perf_counter() - t_setup
+    print(f"setup {setup_s:.1f} s  V={sim.mesh.n_vertices} R=64 dt={sim.time_step_s:.4e}",
+          file=sys.stderr, flush=True)
+    sim.record_er_observations(v, l, w, args.warmup)
+    torch.cuda.synchronize()
+    # timed region: K steps with receiver sampling, graph replays of args.chunk steps
+    t0 = time.perf_counter()
+    traces = sim.record_er_observations(v, l, w, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    assert np.all(np.isfinite(traces))
+    V, R = sim.mesh.n_vertices, len(sim.radial_midpoints_m)
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = args.steps * V * R / elapsed / 1e6
+    print(f"timed {args.steps} steps in {elapsed:.3f} s", file=sys.stderr, flush=True)
+    # per-kernel durations with HIP events on the launch stream
+    h_ms, e_ms = sim.kernel_times_ms(args.kernel_steps)
+    bytes_ = algorithmic_bytes(sim)
+    dominant = "h_step_kernel" if h_ms >= e_ms else "e_step_kernel"
+    dom_ms = max(h_ms, e_ms)
+    dom_bytes = bytes_["h"] if dominant == "h_step_kernel" else bytes_["e"]
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    step_bytes = bytes_["h"] + bytes_["e"]
+    result = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "geodesic FDTD s7 x 64 radial, FP32, exponential loss, "
+                               "Drude plasma >60 km, SI ground, 20 Hz source, 32 receivers",
+                   "cells": V * R, "dof": int(V * (R + 1) + sim.mesh.n_edges * (2 * R + 1)
+                                              + sim.mesh.n_faces * R),
+                   "time_step_s": sim.time_step_s, "graph_chunk": args.chunk},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic(dominant),
+                     "algorithmic_bytes": dom_bytes,
+                     "kernel_ms": {"h_step_kernel": round(h_ms, 5), "e_step_kernel": round(e_ms, 5)},
+                     "step_fraction": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,213 @@
+/*
+ * geofdtd.h -- C ABI of the MI355X (gfx950) GeodesicFDTD time-step library.
+ *
+ * The library advances the four staggered field families of a geodesic
+ * Earth-ionosphere FDTD model (Er on vertices x radial nodes, Ht on edges x
+ * nodes, Et on edges x radial midpoints, Hr on faces x midpoints) with two
+ * fused HIP kernels per leapfrog step.  Everything crossing this boundary is
+ * a plain pointer, size or scalar: host tables are read once by
+ * geo_plan_create (FP64 / int64, as the reference builds them), field state
+ * lives in caller-owned device buffers bound with geo_plan_bind.
+ *
+ * Reference interfaces replaced (paths relative to the ruddyscent/
+ * ionosphere-fdtd source tree, src/ionosphere_fdtd/):
+ *   geo_plan_create   <- GeodesicFDTD.__init__ table setup: solver.py:315-375,
+ *                        _prepare_geometry solver.py:408-442,
+ *                        _prepare_material_coefficients solver.py:706-761,
+ *                        NumPyBackend._vertex_incidence numpy_backend.py:46-76,
+ *                        SurfaceImpedanceADE.__init__ surface_impedance.py:176-203,
+ *                        GeodesicPlasmaCoupler.__init__ plasma.py:458-487,
+ *                        source staggered_distribution sources.py:220-235
+ *   geo_plan_bind     <- the solver's field attributes er/et/hr/ht
+ *                        (solver.py:315-322), SI memory (surface_impedance.py:203),
+ *                        plasma current_density (plasma.py:411); re-callable
+ *                        like checkpoint.py:259-312 replacing them
+ *   geo_step          <- GeodesicFDTD.step (solver.py:789-816) and the compiled
+ *                        chunk path backend.compile_step (solver.py:365-375,
+ *                        torch_backend.py:91-96): _advance_fields
+ *                        (solver.py:867-989) for n steps, full chunks replayed
+ *                        as one HIP graph, remainder as single steps
+ *   geo_sample_er     <- record_er_observations' sample() (solver.py:1122-1124)
+ *   geo_set_receivers <- record_er_observations argument upload
+ *                        (solver.py:1097-1120)
+ *   geo_diagnostics   <- diagnostics() max_abs (solver.py:1027-1030) and the
+ *                        FP64 energy reductions of
+ *                        verification/physics_diagnostics/model.py:242-258
+ *   geo_last_error    <- the Python exceptions the reference raises
+ *
+ * Error convention: 0 = success; GEO_ERR_INVALID maps to ValueError,
+ * GEO_ERR_UNAVAILABLE to BackendUnavailableError (reference base.py:15-16),
+ * GEO_ERR_HIP to RuntimeError.  geo_last_error() returns the message of the
+ * calling thread's last failure.
+ *
+ * Threading: one host thread per plan.  All device work is enqueued on the
+ * caller's stream; no call synchronises the host except geo_diagnostics.
+ */
+#ifndef GEOFDTD_H
+#define GEOFDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GEO_OK 0
+#define GEO_ERR_INVALID (-1)
+#define GEO_ERR_UNAVAILABLE (-2)
+#define GEO_ERR_HIP (-3)
+
+/* Field storage type.  Update arithmetic is FP32 for F32/F16/BF16 and FP64
+ * for F64. */
+#define GEO_STORAGE_F32 0
+#define GEO_STORAGE_F64 1
+#define GEO_STORAGE_F16 2
+#define GEO_STORAGE_BF16 3
+
+#define GEO_MAX_SOURCE 16
+#define GEO_MAX_SI_TERMS 32
+#define GEO_MAX_SPECIES 4
+
+/* Topology and unit-sphere metric of the geodesic mesh (mesh.py:22-41). */
+typedef struct GeoMeshDesc {
+  int64_t n_vertices;
+  int64_t n_edges;
+  int64_t n_faces;
+  const int64_t* edges;            /* (E,2) tail, head */
+  const int64_t* faces;            /* (F,3) */
+  const int64_t* face_edges;       /* (F,3) */
+  const int64_t* face_edge_signs;  /* (F,3) +1 / -1 */
+  const int64_t* edge_left_faces;  /* (E,) */
+  const int64_t* edge_right_faces; /* (E,) */
+  const int64_t* vertex_edges;     /* (V,max_degree) padded incidence */
+  const int64_t* vertex_edge_signs;/* (V,max_degree) +1 tail, -1 head, 0 pad */
+  int64_t max_degree;              /* <= 6 */
+  const double* primal_edge_angles;     /* (E,) theta */
+  const double* dual_edge_angles;       /* (E,) phi */
+  const double* face_solid_angles;      /* (F,) Omega */
+  const double* dual_cell_solid_angles; /* (V,) A */
+} GeoMeshDesc;
+
+/* Radial grid (solver.py:260-283). */
+typedef struct GeoRadialDesc {
+  int64_t radial_cells;      /* R */
+  const double* radii_m;     /* (R+1,) */
+  int32_t full_spherical;    /* 1 full-spherical, 0 thin-shell */
+  int32_t reserved;
+} GeoRadialDesc;
+
+/* Time step, loss coefficients, boundary, sources and plasma. */
+typedef struct GeoPhysicsDesc {
+  int32_t storage;           /* GEO_STORAGE_* */
+  int32_t reserved0;
+  double time_step_s;
+  /* Exponential / trapezoidal loss coefficients.  coef_rows = 1 means one
+   * horizontally uniform radial profile (solver.py:737-748). */
+  const double* ca_er; const double* cb_er; int64_t er_coef_rows; /* (rows,R+1) */
+  const double* ca_et; const double* cb_et; int64_t et_coef_rows; /* (rows,R) */
+  /* Absolute permittivity (F/m) with the same rows as the coefficients, for
+   * the energy diagnostic; NULL means vacuum. */
+  const double* eps_er; const double* eps_et;
+  /* Per-field storage scale factors (powers of two; 1 for F32/F64). */
+  double scale_er, scale_et, scale_hr, scale_ht;
+  /* Surface-impedance lower boundary; si_terms = 0 selects PEC. */
+  int32_t si_terms;
+  int32_t reserved1;
+  const double* si_decay;            /* (terms,) (1-a)/(1+a) */
+  const double* si_drive;            /* (terms,) a/(1+a) */
+  const double* si_history_weights;  /* (terms,) w/(1+a) */
+  const double* si_scale;            /* (si_scale_rows,) sqrt(mu/sigma) */
+  int64_t si_scale_rows;             /* 1 or E */
+  /* Vertical source support (sources.py:220-235). */
+  int32_t n_source;
+  int32_t reserved2;
+  const int64_t* source_vertices;    /* (n,) */
+  const int64_t* source_layers;      /* (n,) radial nodes */
+  const double* source_weights;      /* (n,) */
+  double source_length_m;            /* vertical_element_length_m */
+  /* Tangential source support (sources.py:290-335). */
+  int32_t n_tsource;
+  int32_t reserved3;
+  const int64_t* tsource_edges;
+  const int64_t* tsource_layers;     /* radial midpoints */
+  const double* tsource_weights;
+  /* Cold-plasma ADE (plasma.py:371-527).  n_species = 0 disables it. */
+  int32_t n_species;
+  int32_t magnetized;                /* 0: B == 0 everywhere (Drude form) */
+  int32_t band_lo, band_hi;          /* midpoint layers with plasma state */
+  const double* plasma_reconstruction; /* (F,3,3) pinv(tangents).T */
+  const double* face_centers;        /* (F,3) */
+  const double* left_tangents;       /* (E,3) */
+  const double* right_tangents;      /* (E,3) */
+  const int64_t* vertex_faces;       /* (V,vf_width) */
+  const double* vertex_face_weights; /* (V,vf_width) */
+  int64_t vf_width;                  /* <= 6 */
+  /* ADE coefficients decay, cos, sin, d_parallel, d_real, d_imag for every
+   * species: (n_species, 6, plasma_coef_rows, R); rows = 1 or F. */
+  const double* plasma_coef;
+  int64_t plasma_coef_rows;
+  const double* plasma_b;            /* (plasma_b_rows, R, 3) unit B */
+  int64_t plasma_b_rows;             /* 1 or F */
+} GeoPhysicsDesc;
+
+/* Device buffers owned by the caller.  Pitches count elements. */
+typedef struct GeoState {
+  void* er; int64_t er_pitch;        /* (V, er_pitch), storage type */
+  void* ht; int64_t ht_pitch;        /* (E, ht_pitch) */
+  void* et; int64_t et_pitch;        /* (E, et_pitch) */
+  void* hr; int64_t hr_pitch;        /* (F, hr_pitch) */
+  void* si_memory;                   /* (E, si_terms), compute type */
+  void* plasma_current;              /* (n_species, 3, F, plasma_pitch), compute type */
+  int64_t plasma_pitch;
+} GeoState;
+
+/* Layout a caller must allocate for a plan (all counts in elements). */
+typedef struct GeoLayout {
+  int64_t node_pitch;      /* pitch for Er / Ht rows (>= R+1) */
+  int64_t mid_pitch;       /* pitch for Et / Hr rows (>= R) */
+  int64_t band_lo;         /* first plasma midpoint layer stored */
+  int64_t band_hi;         /* one past the last */
+  int64_t plasma_pitch;    /* pitch of plasma rows (>= band_hi - band_lo) */
+  int64_t vector_elems;    /* elements per 16-byte strip */
+  int64_t compute_bytes;   /* 4 (FP32 arithmetic) or 8 (FP64) */
+} GeoLayout;
+
+typedef struct GeoDiagOut {
+  double max_abs_er, max_abs_et, max_abs_hr, max_abs_ht;
+  double energy_er, energy_et, energy_hr, energy_ht; /* 0.5*eps*E^2*w, 0.5*mu0*H^2*w */
+  int64_t nonfinite;       /* count of non-finite field values */
+} GeoDiagOut;
+
+typedef struct GeoPlan GeoPlan;
+
+int geo_plan_create(const GeoMeshDesc* mesh, const GeoRadialDesc* radial,
+                    const GeoPhysicsDesc* physics, GeoPlan** out);
+int geo_plan_layout(const GeoPlan* plan, GeoLayout* out);
+int geo_plan_bind(GeoPlan* plan, const GeoState* state);
+/* Advance n_steps leapfrog steps.  currents: device array of n_steps source
+ * currents (compute type) evaluated at (n+1/2)dt, or NULL for zero.  chunk > 1
+ * replays full chunks as a HIP graph.  trace: optional device buffer of
+ * (n_steps+1, n_receivers) compute-type rows; row 0 is the state before the
+ * first step (record_er_observations semantics). */
+int geo_step(GeoPlan* plan, const void* currents, int64_t n_steps, int32_t chunk,
+             void* trace, void* stream);
+/* Receivers for trace recording: (n,3) vertices, (n,) node layers, (n,3)
+ * weights. */
+int geo_set_receivers(GeoPlan* plan, const int64_t* vertices, const int64_t* layers,
+                      const double* weights, int32_t n_receivers);
+/* Write one trace row (n_receivers compute-type values) from the current Er. */
+int geo_sample_er(GeoPlan* plan, void* row, void* stream);
+/* FP64 reductions over the bound state; synchronises the stream. */
+int geo_diagnostics(GeoPlan* plan, GeoDiagOut* out, void* stream);
+/* Average kernel durations (ms) of n_steps individually launched and
+ * event-timed steps: out_ms[0] = H kernel, out_ms[1] = E kernel. */
+int geo_time_kernels(GeoPlan* plan, int64_t n_steps, void* stream, double* out_ms);
+void geo_plan_destroy(GeoPlan* plan);
+const char* geo_last_error(void);
+int geo_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GEOFDTD_H */

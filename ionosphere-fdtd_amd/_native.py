@@ -1,0 +1,159 @@
+"""ctypes binding of the C ABI declared in include/geofdtd.h.
+
+The shared library ``libgeofdtd.so`` is built in-tree by ``_build.build()``.
+There is no fallback: if it is missing or no gfx950 device is present, the
+caller gets ``BackendUnavailableError``.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+from .backends.base import BackendUnavailableError
+
+LIBRARY_PATH = Path(__file__).resolve().parent / "libgeofdtd.so"
+
+GEO_OK = 0
+GEO_ERR_INVALID = -1
+GEO_ERR_UNAVAILABLE = -2
+GEO_ERR_HIP = -3
+
+STORAGE_CODES = {"float32": 0, "float64": 1, "float16": 2, "bfloat16": 3}
+
+_p_i64 = C.POINTER(C.c_int64)
+_p_f64 = C.POINTER(C.c_double)
+
+
+class GeoMeshDesc(C.Structure):
+    _fields_ = [
+        ("n_vertices", C.c_int64), ("n_edges", C.c_int64), ("n_faces", C.c_int64),
+        ("edges", _p_i64), ("faces", _p_i64), ("face_edges", _p_i64),
+        ("face_edge_signs", _p_i64), ("edge_left_faces", _p_i64),
+        ("edge_right_faces", _p_i64), ("vertex_edges", _p_i64), ("vertex_edge_signs", _p_i64),
+        ("max_degree", C.c_int64),
+        ("primal_edge_angles", _p_f64), ("dual_edge_angles", _p_f64),
+        ("face_solid_angles", _p_f64), ("dual_cell_solid_angles", _p_f64),
+    ]
+
+
+class GeoRadialDesc(C.Structure):
+    _fields_ = [("radial_cells", C.c_int64), ("radii_m", _p_f64),
+                ("full_spherical", C.c_int32), ("reserved", C.c_int32)]
+
+
+class GeoPhysicsDesc(C.Structure):
+    _fields_ = [
+        ("storage", C.c_int32), ("reserved0", C.c_int32), ("time_step_s", C.c_double),
+        ("ca_er", _p_f64), ("cb_er", _p_f64), ("er_coef_rows", C.c_int64),
+        ("ca_et", _p_f64), ("cb_et", _p_f64), ("et_coef_rows", C.c_int64),
+        ("eps_er", _p_f64), ("eps_et", _p_f64),
+        ("scale_er", C.c_double), ("scale_et", C.c_double), ("scale_hr", C.c_double),
+        ("scale_ht", C.c_double),
+        ("si_terms", C.c_int32), ("reserved1", C.c_int32),
+        ("si_decay", _p_f64), ("si_drive", _p_f64), ("si_history_weights", _p_f64),
+        ("si_scale", _p_f64), ("si_scale_rows", C.c_int64),
+        ("n_source", C.c_int32), ("reserved2", C.c_int32),
+        ("source_vertices", _p_i64), ("source_layers", _p_i64), ("source_weights", _p_f64),
+        ("source_length_m", C.c_double),
+        ("n_tsource", C.c_int32), ("reserved3", C.c_int32),
+        ("tsource_edges", _p_i64), ("tsource_layers", _p_i64), ("tsource_weights", _p_f64),
+        ("n_species", C.c_int32), ("magnetized", C.c_int32),
+        ("band_lo", C.c_int32), ("band_hi", C.c_int32),
+        ("plasma_reconstruction", _p_f64), ("face_centers", _p_f64),
+        ("left_tangents", _p_f64), ("right_tangents", _p_f64),
+        ("vertex_faces", _p_i64), ("vertex_face_weights", _p_f64), ("vf_width", C.c_int64),
+        ("plasma_coef", _p_f64), ("plasma_coef_rows", C.c_int64),
+        ("plasma_b", _p_f64), ("plasma_b_rows", C.c_int64),
+    ]
+
+
+class GeoState(C.Structure):
+    _fields_ = [
+        ("er", C.c_void_p), ("er_pitch", C.c_int64), ("ht", C.c_void_p), ("ht_pitch", C.c_int64),
+        ("et", C.c_void_p), ("et_pitch", C.c_int64), ("hr", C.c_void_p), ("hr_pitch", C.c_int64),
+        ("si_memory", C.c_void_p), ("plasma_current", C.c_void_p), ("plasma_pitch", C.c_int64),
+    ]
+
+
+class GeoLayout(C.Structure):
+    _fields_ = [("node_pitch", C.c_int64), ("mid_pitch", C.c_int64), ("band_lo", C.c_int64),
+                ("band_hi", C.c_int64), ("plasma_pitch", C.c_int64),
+                ("vector_elems", C.c_int64), ("compute_bytes", C.c_int64)]
+
+
+class GeoDiagOut(C.Structure):
+    _fields_ = [("max_abs_er", C.c_double), ("max_abs_et", C.c_double),
+                ("max_abs_hr", C.c_double), ("max_abs_ht", C.c_double),
+                ("energy_er", C.c_double), ("energy_et", C.c_double),
+                ("energy_hr", C.c_double), ("energy_ht", C.c_double),
+                ("nonfinite", C.c_int64)]
+
+
+EXPORTED_SYMBOLS = (
+    "geo_plan_create", "geo_plan_layout", "geo_plan_bind", "geo_step", "geo_set_receivers",
+    "geo_sample_er", "geo_diagnostics", "geo_time_kernels", "geo_plan_destroy",
+    "geo_last_error", "geo_version",
+)
+
+_lib = None
+
+
+def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
+    """Load libgeofdtd.so (no device call happens here)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    target = Path(path) if path is not None else LIBRARY_PATH
+    if not target.exists():
+        raise BackendUnavailableError(
+            f"HIP extension {target} is not built; run __graft_entry__.build()")
+    lib = C.CDLL(str(target))
+    vp = C.c_void_p
+    lib.geo_plan_create.argtypes = [C.POINTER(GeoMeshDesc), C.POINTER(GeoRadialDesc),
+                                    C.POINTER(GeoPhysicsDesc), C.POINTER(vp)]
+    lib.geo_plan_layout.argtypes = [vp, C.POINTER(GeoLayout)]
+    lib.geo_plan_bind.argtypes = [vp, C.POINTER(GeoState)]
+    lib.geo_step.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp]
+    lib.geo_set_receivers.argtypes = [vp, _p_i64, _p_i64, _p_f64, C.c_int32]
+    lib.geo_sample_er.argtypes = [vp, vp, vp]
+    lib.geo_diagnostics.argtypes = [vp, C.POINTER(GeoDiagOut), vp]
+    lib.geo_time_kernels.argtypes = [vp, C.c_int64, vp, _p_f64]
+    lib.geo_plan_destroy.argtypes = [vp]
+    lib.geo_plan_destroy.restype = None
+    lib.geo_last_error.restype = C.c_char_p
+    lib.geo_last_error.argtypes = []
+    lib.geo_version.restype = C.c_int
+    for name in EXPORTED_SYMBOLS:
+        if name not in ("geo_plan_destroy", "geo_last_error", "geo_version"):
+            getattr(lib, name).restype = C.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def check(code: int, lib: C.CDLL | None = None) -> None:
+    """Raise the Python exception matching a C ABI error code."""
+    if code == GEO_OK:
+        return
+    lib = lib or load_library()
+    message = (lib.geo_last_error() or b"").decode(errors="replace")
+    if code == GEO_ERR_INVALID:
+        raise ValueError(message)
+    if code == GEO_ERR_UNAVAILABLE:
+        raise BackendUnavailableError(message)
+    raise RuntimeError(message)
+
+
+def f64_ptr(array):
+    """ctypes double* into a C-contiguous float64 numpy array (None -> NULL)."""
+    if array is None:
+        return None
+    return array.ctypes.data_as(_p_f64)
+
+
+def i64_ptr(array):
+    if array is None:
+        return None
+    return array.ctypes.data_as(_p_i64)

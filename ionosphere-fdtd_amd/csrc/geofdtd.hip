@@ -1,0 +1,929 @@
+// C ABI of the GeodesicFDTD step library (declared in include/geofdtd.h).
+//
+// geo_plan_create turns the reference's FP64 host tables into the device
+// tables the two step kernels read (int32 indices, compute-type metric and
+// loss coefficients folded with dt), builds the XCD-aware block maps, and
+// owns them until geo_plan_destroy.  Field state stays in caller buffers.
+#include "geofdtd.h"
+#include "geofdtd_kernels.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace {
+
+constexpr double kMu0 = 1.25663706212e-6;  // constants.py:4
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define GEO_HIP(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      return fail(GEO_ERR_HIP, std::string(#expr) + " failed: " + hipGetErrorString(_e)); \
+  } while (0)
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// Owns device allocations made at plan creation.
+struct DeviceArena {
+  std::vector<void*> blocks;
+  ~DeviceArena() {
+    for (void* p : blocks) (void)hipFree(p);
+  }
+  template <typename X>
+  int upload(const std::vector<X>& host, X** out) {
+    *out = nullptr;
+    if (host.empty()) return GEO_OK;
+    void* p = nullptr;
+    GEO_HIP(hipMalloc(&p, host.size() * sizeof(X)));
+    blocks.push_back(p);
+    GEO_HIP(hipMemcpy(p, host.data(), host.size() * sizeof(X), hipMemcpyHostToDevice));
+    *out = static_cast<X*>(p);
+    return GEO_OK;
+  }
+  int zeroed(size_t bytes, void** out) {
+    void* p = nullptr;
+    GEO_HIP(hipMalloc(&p, bytes));
+    blocks.push_back(p);
+    GEO_HIP(hipMemset(p, 0, bytes));
+    *out = p;
+    return GEO_OK;
+  }
+};
+
+struct SegSpec {
+  uint32_t segment;
+  int32_t blocks;
+};
+
+// Interleave the blocks of several segments in proportion to their sizes so
+// that, e.g., edge and face blocks covering the same mesh region run close
+// together; entry = segment << 24 | block index within the segment.
+std::vector<uint32_t> build_block_map(const std::vector<SegSpec>& segs) {
+  struct Item { double key; uint32_t seg_order; uint32_t value; };
+  std::vector<Item> items;
+  uint32_t order = 0;
+  for (const SegSpec& s : segs) {
+    for (int32_t i = 0; i < s.blocks; ++i)
+      items.push_back({(i + 0.5) / s.blocks, order, (s.segment << 24) | (uint32_t)i});
+    ++order;
+  }
+  std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+    return a.key < b.key || (a.key == b.key && a.seg_order < b.seg_order);
+  });
+  std::vector<uint32_t> map(items.size());
+  for (size_t i = 0; i < items.size(); ++i) map[i] = items[i].value;
+  return map;
+}
+
+geo::SegGeom make_geom(int64_t rows, int64_t pitch, int64_t strip_elems) {
+  geo::SegGeom g;
+  g.n_rows = (int32_t)rows;
+  g.strips = (int32_t)(pitch / strip_elems);
+  g.rows_per_block = geo::kBlock / g.strips;
+  g.magic = (uint32_t)((65536 + g.strips - 1) / g.strips);
+  return g;
+}
+
+int32_t geom_blocks(const geo::SegGeom& g) {
+  return (int32_t)((g.n_rows + g.rows_per_block - 1) / g.rows_per_block);
+}
+
+struct PlanBase {
+  virtual ~PlanBase() = default;
+  virtual int bind(const GeoState* s) = 0;
+  virtual int step(const void* currents, int64_t n, int32_t chunk, void* trace,
+                   hipStream_t stream) = 0;
+  virtual int set_receivers(const int64_t* v, const int64_t* l, const double* w, int32_t n) = 0;
+  virtual int sample(void* row, hipStream_t stream) = 0;
+  virtual int diagnostics(GeoDiagOut* out, hipStream_t stream) = 0;
+  virtual int time_kernels(int64_t n, hipStream_t stream, double* out_ms) = 0;
+  GeoLayout layout{};
+};
+
+// FP64 reduction kernels for diagnostics: per-block partials, then one block.
+struct DiagAcc {
+  double max_abs;
+  double energy;
+  long long nonfinite;
+};
+
+template <typename T, typename C>
+__global__ void diag_field_kernel(const T* __restrict__ x, int64_t rows, int32_t cols,
+                                  int32_t pitch, C inv_scale, const double* __restrict__ row_w,
+                                  const double* __restrict__ col_w,
+                                  const double* __restrict__ eps, int64_t eps_stride,
+                                  double coef, DiagAcc* __restrict__ partial) {
+  __shared__ double s_max[geo::kBlock];
+  __shared__ double s_en[geo::kBlock];
+  __shared__ long long s_nf[geo::kBlock];
+  double mx = 0.0, en = 0.0;
+  long long nf = 0;
+  const int64_t total = rows * cols;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / cols;
+    const int32_t c = (int32_t)(i - r * cols);
+    const double v = (double)geo::load_one<T, C>(x + r * pitch + c, inv_scale);
+    if (!isfinite(v)) {
+      ++nf;
+      continue;
+    }
+    mx = fmax(mx, fabs(v));
+    const double e = eps != nullptr ? eps[r * eps_stride + c] : 1.0;
+    en += coef * e * v * v * row_w[r] * col_w[c];
+  }
+  s_max[threadIdx.x] = mx;
+  s_en[threadIdx.x] = en;
+  s_nf[threadIdx.x] = nf;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + s]);
+      s_en[threadIdx.x] += s_en[threadIdx.x + s];
+      s_nf[threadIdx.x] += s_nf[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = {s_max[0], s_en[0], s_nf[0]};
+}
+
+__global__ void diag_reduce_kernel(const DiagAcc* __restrict__ partial, int32_t n,
+                                   DiagAcc* __restrict__ out) {
+  __shared__ double s_max[geo::kBlock];
+  __shared__ double s_en[geo::kBlock];
+  __shared__ long long s_nf[geo::kBlock];
+  double mx = 0.0, en = 0.0;
+  long long nf = 0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    mx = fmax(mx, partial[i].max_abs);
+    en += partial[i].energy;
+    nf += partial[i].nonfinite;
+  }
+  s_max[threadIdx.x] = mx;
+  s_en[threadIdx.x] = en;
+  s_nf[threadIdx.x] = nf;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + s]);
+      s_en[threadIdx.x] += s_en[threadIdx.x + s];
+      s_nf[threadIdx.x] += s_nf[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = {s_max[0], s_en[0], s_nf[0]};
+}
+
+template <typename T, typename C>
+struct Plan final : PlanBase {
+  static constexpr int N = geo::Strip<T>::N;
+  DeviceArena arena;
+  int64_t V = 0, E = 0, F = 0, R = 0, pn = 0, pm = 0;
+  bool plasma = false, magnetized = false, si_on = false;
+  int32_t si_terms = 0;
+  geo::HParams<T, C> hp{};
+  geo::EParams<T, C> ep{};
+  int32_t h_grid = 0, e_grid = 0;
+  geo::Dynamic* dyn = nullptr;
+  bool bound = false;
+  // receivers (owned separately so repeated record calls do not grow the arena)
+  int32_t n_rx = 0;
+  void* rx_mem = nullptr;
+  // permittivity for energy diagnostics (absolute, F/m), rows 1 or entity count
+  double* eps_er_dev = nullptr;
+  double* eps_et_dev = nullptr;
+  int64_t eps_er_stride = 0, eps_et_stride = 0;
+  // graph cache
+  hipStream_t capture_stream = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int32_t graph_chunk = 0;
+  // diagnostics weights (double, device)
+  double *w_er_row = nullptr, *w_er_col = nullptr, *w_et_row = nullptr, *w_et_col = nullptr;
+  double *w_hr_row = nullptr, *w_hr_col = nullptr, *w_ht_row = nullptr, *w_ht_col = nullptr;
+  DiagAcc* diag_partial = nullptr;
+  DiagAcc* diag_out = nullptr;
+  static constexpr int kDiagBlocks = 1024;
+
+  ~Plan() override {
+    if (rx_mem) (void)hipFree(rx_mem);
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+  }
+
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_chunk = 0;
+  }
+
+  int create(const GeoMeshDesc* m, const GeoRadialDesc* r, const GeoPhysicsDesc* ph) {
+    V = m->n_vertices;
+    E = m->n_edges;
+    F = m->n_faces;
+    R = r->radial_cells;
+    if (V < 3 || E < 3 || F < 2 || R < 2)
+      return fail(GEO_ERR_INVALID, "mesh and radial sizes must be positive");
+    if (E >= (1LL << 30) || V * (R + N) >= (1LL << 40))
+      return fail(GEO_ERR_INVALID, "mesh is too large for 32-bit entity indices");
+    if (m->max_degree < 1 || m->max_degree > 6)
+      return fail(GEO_ERR_INVALID, "vertex degree must be at most 6");
+    pn = round_up(R + 1, N);
+    pm = round_up(R, N);
+    const double dt = ph->time_step_s;
+    if (!(dt > 0.0) || !std::isfinite(dt))
+      return fail(GEO_ERR_INVALID, "time step must be finite and positive");
+    const double dtm = dt / kMu0;
+    // ---- radial tables (solver.py:260-283, 443-465) ----
+    std::vector<double> rad(r->radii_m, r->radii_m + R + 1);
+    for (int64_t k = 0; k < R; ++k)
+      if (!(rad[k + 1] > rad[k])) return fail(GEO_ERR_INVALID, "radii must increase");
+    std::vector<double> mid(R), dr(R), ctrl(R + 1);
+    for (int64_t j = 0; j < R; ++j) {
+      mid[j] = 0.5 * (rad[j] + rad[j + 1]);
+      dr[j] = rad[j + 1] - rad[j];
+    }
+    ctrl[0] = 0.5 * dr[0];
+    ctrl[R] = 0.5 * dr[R - 1];
+    for (int64_t k = 1; k < R; ++k) ctrl[k] = mid[k] - mid[k - 1];
+    const bool full = r->full_spherical != 0;
+    std::vector<C> inv_r(pn, 0), rd_b(pn, 0), rd_c(pn, 0), inv_rbar(pm, 0), rh_a(pm, 0),
+        rh_c(pm, 0);
+    for (int64_t k = 0; k <= R; ++k) {
+      inv_r[k] = (C)(1.0 / rad[k]);
+      double alpha, gamma = 0.0;
+      if (k == 0) alpha = full ? 2.0 * mid[0] / (dr[0] * rad[0]) : 2.0 / dr[0];
+      else if (k == R) alpha = full ? 2.0 * mid[R - 1] / (dr[R - 1] * rad[R]) : 2.0 / dr[R - 1];
+      else {
+        const double d = mid[k] - mid[k - 1];
+        alpha = full ? mid[k - 1] / (d * rad[k]) : 1.0 / d;
+        gamma = full ? 1.0 / rad[k] : 0.0;
+      }
+      rd_b[k] = (C)(dtm * alpha);
+      rd_c[k] = (C)(dtm * gamma);
+    }
+    for (int64_t j = 0; j < R; ++j) {
+      inv_rbar[j] = (C)(1.0 / mid[j]);
+      rh_a[j] = (C)(full ? rad[j] / (dr[j] * mid[j]) : 1.0 / dr[j]);
+      rh_c[j] = (C)(full ? 1.0 / mid[j] : 0.0);
+    }
+    // ---- topology and metric (solver.py:408-442, numpy_backend.py:46-114) ----
+    std::vector<int32_t> edges(2 * E), edge_lr(2 * E), face_edges(3 * F), faces(3 * F);
+    std::vector<C> edge_grad(E), edge_dual(E), face_curl(3 * F);
+    for (int64_t e = 0; e < E; ++e) {
+      const int64_t t = m->edges[2 * e], h = m->edges[2 * e + 1];
+      const int64_t l = m->edge_left_faces[e], rr = m->edge_right_faces[e];
+      if (t < 0 || t >= V || h < 0 || h >= V || l < 0 || l >= F || rr < 0 || rr >= F)
+        return fail(GEO_ERR_INVALID, "edge incidence index out of range");
+      edges[2 * e] = (int32_t)t;
+      edges[2 * e + 1] = (int32_t)h;
+      edge_lr[2 * e] = (int32_t)l;
+      edge_lr[2 * e + 1] = (int32_t)rr;
+      const double theta = m->primal_edge_angles[e], phi = m->dual_edge_angles[e];
+      if (!(theta > 0.0) || !(phi > 0.0)) return fail(GEO_ERR_INVALID, "degenerate edge");
+      edge_grad[e] = (C)(dtm / theta);
+      edge_dual[e] = (C)(1.0 / phi);
+    }
+    for (int64_t f = 0; f < F; ++f) {
+      const double omega = m->face_solid_angles[f];
+      if (!(omega > 0.0)) return fail(GEO_ERR_INVALID, "degenerate face");
+      for (int c = 0; c < 3; ++c) {
+        const int64_t e = m->face_edges[3 * f + c];
+        const int64_t v = m->faces[3 * f + c];
+        if (e < 0 || e >= E || v < 0 || v >= V)
+          return fail(GEO_ERR_INVALID, "face incidence index out of range");
+        face_edges[3 * f + c] = (int32_t)e;
+        faces[3 * f + c] = (int32_t)v;
+        face_curl[3 * f + c] =
+            (C)(dtm * (double)m->face_edge_signs[3 * f + c] * m->primal_edge_angles[e] / omega);
+      }
+    }
+    std::vector<int32_t> vedges(6 * V, 0);
+    std::vector<C> vw(6 * V, 0);
+    for (int64_t v = 0; v < V; ++v) {
+      const double area = m->dual_cell_solid_angles[v];
+      if (!(area > 0.0)) return fail(GEO_ERR_INVALID, "degenerate dual cell");
+      for (int64_t s = 0; s < m->max_degree; ++s) {
+        const int64_t e = m->vertex_edges[v * m->max_degree + s];
+        const int64_t sg = m->vertex_edge_signs[v * m->max_degree + s];
+        if (e < 0 || e >= E) return fail(GEO_ERR_INVALID, "vertex incidence out of range");
+        vedges[6 * v + s] = (int32_t)e;
+        vw[6 * v + s] = (C)((double)sg * m->dual_edge_angles[e] / area);
+      }
+      for (int64_t s = m->max_degree; s < 6; ++s) vedges[6 * v + s] = vedges[6 * v];
+    }
+    // ---- loss coefficients ----
+    const int64_t er_rows = ph->er_coef_rows, et_rows = ph->et_coef_rows;
+    if (!(er_rows == 1 || er_rows == V) || !(et_rows == 1 || et_rows == E))
+      return fail(GEO_ERR_INVALID, "coefficient rows must be 1 or the entity count");
+    std::vector<C> ca_er(er_rows * pn, 0), cb_er(er_rows * pn, 0), ca_et(et_rows * pm, 0),
+        cb_et(et_rows * pm, 0);
+    for (int64_t i = 0; i < er_rows; ++i)
+      for (int64_t k = 0; k <= R; ++k) {
+        ca_er[i * pn + k] = (C)ph->ca_er[i * (R + 1) + k];
+        cb_er[i * pn + k] = (C)ph->cb_er[i * (R + 1) + k];
+      }
+    for (int64_t i = 0; i < et_rows; ++i)
+      for (int64_t j = 0; j < R; ++j) {
+        ca_et[i * pm + j] = (C)ph->ca_et[i * R + j];
+        cb_et[i * pm + j] = (C)ph->cb_et[i * R + j];
+      }
+    const bool narrow = sizeof(T) == 2;
+    const double s_er = narrow ? ph->scale_er : 1.0, s_et = narrow ? ph->scale_et : 1.0;
+    const double s_hr = narrow ? ph->scale_hr : 1.0, s_ht = narrow ? ph->scale_ht : 1.0;
+    // ---- surface impedance ----
+    si_terms = ph->si_terms;
+    si_on = si_terms > 0;
+    if (si_terms < 0 || si_terms > geo::kMaxSiTerms)
+      return fail(GEO_ERR_INVALID, "surface impedance terms out of range");
+    std::vector<C> si_scale;
+    if (si_on) {
+      if (!(ph->si_scale_rows == 1 || ph->si_scale_rows == E))
+        return fail(GEO_ERR_INVALID, "surface impedance scale rows must be 1 or E");
+      si_scale.assign(ph->si_scale, ph->si_scale + ph->si_scale_rows);
+    }
+    // ---- sources ----
+    if (ph->n_source < 0 || ph->n_source > geo::kMaxSource || ph->n_tsource < 0 ||
+        ph->n_tsource > geo::kMaxSource)
+      return fail(GEO_ERR_INVALID, "too many source support points");
+    // ---- plasma ----
+    plasma = ph->n_species > 0;
+    magnetized = plasma && ph->magnetized != 0;
+    int64_t band_lo = 0, band_hi = 0, ppitch = 0;
+    std::vector<C> recon, center, ltan, rtan, vfw, coef, bdir;
+    std::vector<int32_t> vfaces;
+    if (plasma) {
+      if (ph->n_species > geo::kMaxSpecies) return fail(GEO_ERR_INVALID, "too many species");
+      if (ph->band_lo < 0 || ph->band_hi > R || ph->band_lo >= ph->band_hi)
+        return fail(GEO_ERR_INVALID, "plasma band is invalid");
+      band_lo = ph->band_lo / N * N;
+      band_hi = ph->band_hi;
+      ppitch = round_up(band_hi - band_lo, N);
+      recon.assign(ph->plasma_reconstruction, ph->plasma_reconstruction + 9 * F);
+      center.assign(ph->face_centers, ph->face_centers + 3 * F);
+      ltan.assign(ph->left_tangents, ph->left_tangents + 3 * E);
+      rtan.assign(ph->right_tangents, ph->right_tangents + 3 * E);
+      if (ph->vf_width < 1 || ph->vf_width > 6)
+        return fail(GEO_ERR_INVALID, "vertex-face width must be at most 6");
+      vfaces.assign(6 * V, 0);
+      vfw.assign(6 * V, 0);
+      for (int64_t v = 0; v < V; ++v)
+        for (int64_t s = 0; s < ph->vf_width; ++s) {
+          const int64_t f = ph->vertex_faces[v * ph->vf_width + s];
+          if (f < 0 || f >= F) return fail(GEO_ERR_INVALID, "vertex face out of range");
+          vfaces[6 * v + s] = (int32_t)f;
+          vfw[6 * v + s] = (C)ph->vertex_face_weights[v * ph->vf_width + s];
+        }
+      const int64_t rows = ph->plasma_coef_rows;
+      if (!(rows == 1 || rows == F)) return fail(GEO_ERR_INVALID, "plasma rows must be 1 or F");
+      coef.assign((size_t)ph->n_species * 6 * rows * ppitch, 0);
+      for (int64_t s = 0; s < ph->n_species; ++s)
+        for (int q = 0; q < 6; ++q)
+          for (int64_t i = 0; i < rows; ++i)
+            for (int64_t j = band_lo; j < band_hi; ++j)
+              coef[((s * 6 + q) * rows + i) * ppitch + (j - band_lo)] =
+                  (C)ph->plasma_coef[((s * 6 + q) * rows + i) * R + j];
+      if (magnetized) {
+        const int64_t brows = ph->plasma_b_rows;
+        if (!(brows == 1 || brows == F)) return fail(GEO_ERR_INVALID, "B rows must be 1 or F");
+        bdir.assign((size_t)brows * ppitch * 3, 0);
+        for (int64_t i = 0; i < brows; ++i)
+          for (int64_t j = band_lo; j < band_hi; ++j)
+            for (int d = 0; d < 3; ++d)
+              bdir[(i * ppitch + (j - band_lo)) * 3 + d] = (C)ph->plasma_b[(i * R + j) * 3 + d];
+      }
+    }
+    // ---- upload ----
+    int rc;
+#define UP(vec, ptr) \
+  if ((rc = arena.upload(vec, &ptr)) != GEO_OK) return rc
+    int32_t *d_edges, *d_lr, *d_fe, *d_faces, *d_ve, *d_vf = nullptr;
+    C *d_eg, *d_ed, *d_fc, *d_vw, *d_ir, *d_rdb, *d_rdc, *d_irb, *d_rha, *d_rhc;
+    C *d_caer, *d_cber, *d_caet, *d_cbet, *d_sis = nullptr;
+    C *d_recon = nullptr, *d_center = nullptr, *d_lt = nullptr, *d_rt = nullptr;
+    C *d_vfw = nullptr, *d_coef = nullptr, *d_b = nullptr;
+    UP(edges, d_edges); UP(edge_lr, d_lr); UP(face_edges, d_fe); UP(faces, d_faces);
+    UP(vedges, d_ve); UP(edge_grad, d_eg); UP(edge_dual, d_ed); UP(face_curl, d_fc);
+    UP(vw, d_vw); UP(inv_r, d_ir); UP(rd_b, d_rdb); UP(rd_c, d_rdc); UP(inv_rbar, d_irb);
+    UP(rh_a, d_rha); UP(rh_c, d_rhc); UP(ca_er, d_caer); UP(cb_er, d_cber);
+    UP(ca_et, d_caet); UP(cb_et, d_cbet);
+    if (si_on) UP(si_scale, d_sis);
+    if (plasma) {
+      UP(recon, d_recon); UP(center, d_center); UP(ltan, d_lt); UP(rtan, d_rt);
+      UP(vfaces, d_vf); UP(vfw, d_vfw); UP(coef, d_coef);
+      if (magnetized) UP(bdir, d_b);
+    }
+#undef UP
+    void* dyn_mem = nullptr;
+    if ((rc = arena.zeroed(sizeof(geo::Dynamic), &dyn_mem)) != GEO_OK) return rc;
+    dyn = static_cast<geo::Dynamic*>(dyn_mem);
+
+    geo::PlasmaTables<C> pl{};
+    if (plasma) {
+      pl.n_species = ph->n_species;
+      pl.band_lo = (int32_t)band_lo;
+      pl.band_hi = (int32_t)band_hi;
+      pl.pitch = (int32_t)ppitch;
+      pl.comp_stride = F * ppitch;
+      pl.species_stride = 3 * F * ppitch;
+      pl.recon = d_recon;
+      pl.center = d_center;
+      pl.left_tan = d_lt;
+      pl.right_tan = d_rt;
+      pl.vertex_faces = d_vf;
+      pl.vertex_face_w = d_vfw;
+      pl.coef = d_coef;
+      pl.coef_row_stride = ph->plasma_coef_rows == 1 ? 0 : ppitch;
+      pl.coef_q_stride = ph->plasma_coef_rows * ppitch;
+      pl.coef_species_stride = 6 * ph->plasma_coef_rows * ppitch;
+      pl.bdir = d_b;
+      pl.b_row_stride = (magnetized && ph->plasma_b_rows == F) ? 3 * ppitch : 0;
+    }
+    // ---- H kernel parameters and block map ----
+    hp.pn = (int32_t)pn;
+    hp.pm = (int32_t)pm;
+    hp.R = (int32_t)R;
+    hp.si_on = si_on ? 1 : 0;
+    hp.s_er_inv = (C)(1.0 / s_er);
+    hp.s_et_inv = (C)(1.0 / s_et);
+    hp.s_ht = (C)s_ht;
+    hp.s_ht_inv = (C)(1.0 / s_ht);
+    hp.s_hr = (C)s_hr;
+    hp.s_hr_inv = (C)(1.0 / s_hr);
+    hp.edges = d_edges;
+    hp.edge_grad = d_eg;
+    hp.face_edges = d_fe;
+    hp.faces = d_faces;
+    hp.face_curl = d_fc;
+    hp.inv_r = d_ir;
+    hp.rd_b = d_rdb;
+    hp.rd_c = d_rdc;
+    hp.inv_rbar = d_irb;
+    hp.si_scale = d_sis;
+    hp.si_scale_stride = (si_on && ph->si_scale_rows == E) ? 1 : 0;
+    hp.si_terms = si_terms;
+    if (si_on) {
+      const double cs = 2.0 * dt / (kMu0 * dr[0]);
+      const double metric = full ? mid[0] / rad[0] : 1.0;
+      hp.si_cs = (C)cs;
+      hp.si_cs_metric = (C)(cs * metric);
+      double hsum = 0.0;
+      for (int i = 0; i < si_terms; ++i) {
+        hp.si_decay[i] = (C)ph->si_decay[i];
+        hp.si_drive[i] = (C)ph->si_drive[i];
+        hp.si_hw[i] = (C)ph->si_history_weights[i];
+        hsum += (double)(C)ph->si_history_weights[i];
+      }
+      hp.si_hw_sum = (C)hsum;
+    }
+    hp.pl = pl;
+    hp.dyn = dyn;
+    hp.n_edges = (int32_t)E;
+    hp.g_edge = make_geom(E, pn, N);
+    hp.g_face = make_geom(F, pm, N);
+    {
+      std::vector<SegSpec> segs = {{geo::kSegHEdge, geom_blocks(hp.g_edge)},
+                                   {geo::kSegHFace, geom_blocks(hp.g_face)}};
+      if (si_on) segs.push_back({geo::kSegSi, (int32_t)((E + geo::kBlock - 1) / geo::kBlock)});
+      segs.push_back({geo::kSegSample, 1});
+      std::vector<uint32_t> map = build_block_map(segs);
+      uint32_t* d_map;
+      if ((rc = arena.upload(map, &d_map)) != GEO_OK) return rc;
+      hp.block_map = d_map;
+      hp.n_blocks = (int32_t)map.size();
+      hp.per_xcd = (hp.n_blocks + geo::kNumXcd - 1) / geo::kNumXcd;
+      h_grid = hp.per_xcd * geo::kNumXcd;
+    }
+    // ---- E kernel parameters and block map ----
+    ep.pn = (int32_t)pn;
+    ep.pm = (int32_t)pm;
+    ep.R = (int32_t)R;
+    ep.ca_er_stride = er_rows == 1 ? 0 : (int32_t)pn;
+    ep.ca_et_stride = et_rows == 1 ? 0 : (int32_t)pm;
+    ep.s_er = (C)s_er;
+    ep.s_er_inv = (C)(1.0 / s_er);
+    ep.s_et = (C)s_et;
+    ep.s_et_inv = (C)(1.0 / s_et);
+    ep.s_ht_inv = (C)(1.0 / s_ht);
+    ep.s_hr_inv = (C)(1.0 / s_hr);
+    ep.vertex_edges = d_ve;
+    ep.vertex_w = d_vw;
+    ep.edge_lr = d_lr;
+    ep.edge_dual = d_ed;
+    ep.ca_er = d_caer;
+    ep.cb_er = d_cber;
+    ep.ca_et = d_caet;
+    ep.cb_et = d_cbet;
+    ep.inv_r = d_ir;
+    ep.inv_rbar = d_irb;
+    ep.rh_a = d_rha;
+    ep.rh_c = d_rhc;
+    ep.pl = pl;
+    ep.n_src = ph->n_source;
+    for (int q = 0; q < ph->n_source; ++q) {
+      const int64_t v = ph->source_vertices[q], l = ph->source_layers[q];
+      if (v < 0 || v >= V || l < 0 || l > R)
+        return fail(GEO_ERR_INVALID, "source support index out of range");
+      ep.src_row[q] = (int32_t)v;
+      ep.src_layer[q] = (int32_t)l;
+      // solver.py:936-945: w * l / (A_v r_k^2 L_k)
+      ep.src_factor[q] = (C)(ph->source_weights[q] * ph->source_length_m /
+                             m->dual_cell_solid_angles[v] / (rad[l] * rad[l]) / ctrl[l]);
+    }
+    ep.n_tsrc = ph->n_tsource;
+    for (int q = 0; q < ph->n_tsource; ++q) {
+      const int64_t e = ph->tsource_edges[q], l = ph->tsource_layers[q];
+      if (e < 0 || e >= E || l < 0 || l >= R)
+        return fail(GEO_ERR_INVALID, "tangential source index out of range");
+      ep.tsrc_row[q] = (int32_t)e;
+      ep.tsrc_layer[q] = (int32_t)l;
+      // solver.py:973-981: w / (phi_e rbar_j dr_j)
+      ep.tsrc_factor[q] =
+          (C)(ph->tsource_weights[q] / m->dual_edge_angles[e] / mid[l] / dr[l]);
+    }
+    ep.dyn = dyn;
+    ep.g_vertex = make_geom(V, pn, N);
+    ep.g_edge = make_geom(E, pm, N);
+    {
+      std::vector<SegSpec> segs = {{geo::kSegEVertex, geom_blocks(ep.g_vertex)},
+                                   {geo::kSegEEdge, geom_blocks(ep.g_edge)}};
+      std::vector<uint32_t> map = build_block_map(segs);
+      uint32_t* d_map;
+      if ((rc = arena.upload(map, &d_map)) != GEO_OK) return rc;
+      ep.block_map = d_map;
+      ep.n_blocks = (int32_t)map.size();
+      ep.per_xcd = (ep.n_blocks + geo::kNumXcd - 1) / geo::kNumXcd;
+      e_grid = ep.per_xcd * geo::kNumXcd;
+    }
+    // ---- diagnostics weights (physics_diagnostics/model.py:78-92) ----
+    {
+      std::vector<double> eer, eet;
+      if (ph->eps_er) eer.assign(ph->eps_er, ph->eps_er + er_rows * (R + 1));
+      else eer.assign(R + 1, 8.8541878128e-12);
+      if (ph->eps_et) eet.assign(ph->eps_et, ph->eps_et + et_rows * R);
+      else eet.assign(R, 8.8541878128e-12);
+      if ((rc = arena.upload(eer, &eps_er_dev)) != GEO_OK) return rc;
+      if ((rc = arena.upload(eet, &eps_et_dev)) != GEO_OK) return rc;
+      eps_er_stride = (ph->eps_er && er_rows > 1) ? R + 1 : 0;
+      eps_et_stride = (ph->eps_et && et_rows > 1) ? R : 0;
+      std::vector<double> er_row(V), er_col(R + 1), et_row(E), et_col(R), hr_row(F), hr_col(R),
+          ht_row(E), ht_col(R + 1);
+      for (int64_t v = 0; v < V; ++v) er_row[v] = m->dual_cell_solid_angles[v];
+      for (int64_t e = 0; e < E; ++e)
+        et_row[e] = ht_row[e] = m->primal_edge_angles[e] * m->dual_edge_angles[e];
+      for (int64_t f = 0; f < F; ++f) hr_row[f] = m->face_solid_angles[f];
+      for (int64_t k = 0; k <= R; ++k) er_col[k] = ht_col[k] = rad[k] * rad[k] * ctrl[k];
+      for (int64_t j = 0; j < R; ++j) et_col[j] = hr_col[j] = mid[j] * mid[j] * dr[j];
+      if ((rc = arena.upload(er_row, &w_er_row)) != GEO_OK) return rc;
+      if ((rc = arena.upload(er_col, &w_er_col)) != GEO_OK) return rc;
+      if ((rc = arena.upload(et_row, &w_et_row)) != GEO_OK) return rc;
+      if ((rc = arena.upload(et_col, &w_et_col)) != GEO_OK) return rc;
+      if ((rc = arena.upload(hr_row, &w_hr_row)) != GEO_OK) return rc;
+      if ((rc = arena.upload(hr_col, &w_hr_col)) != GEO_OK) return rc;
+      if ((rc = arena.upload(ht_row, &w_ht_row)) != GEO_OK) return rc;
+      if ((rc = arena.upload(ht_col, &w_ht_col)) != GEO_OK) return rc;
+      void* p = nullptr;
+      if ((rc = arena.zeroed(sizeof(DiagAcc) * (kDiagBlocks * 4 + 4), &p)) != GEO_OK) return rc;
+      diag_partial = static_cast<DiagAcc*>(p);
+      diag_out = diag_partial + kDiagBlocks * 4;
+    }
+    layout.node_pitch = pn;
+    layout.mid_pitch = pm;
+    layout.band_lo = band_lo;
+    layout.band_hi = band_hi;
+    layout.plasma_pitch = ppitch;
+    layout.vector_elems = N;
+    layout.compute_bytes = sizeof(C);
+    GEO_HIP(hipStreamCreateWithFlags(&capture_stream, hipStreamNonBlocking));
+    return GEO_OK;
+  }
+
+  static bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+  int bind(const GeoState* s) override {
+    if (s->er_pitch != pn || s->ht_pitch != pn || s->et_pitch != pm || s->hr_pitch != pm)
+      return fail(GEO_ERR_INVALID, "field row pitches do not match the plan layout");
+    if (!s->er || !s->ht || !s->et || !s->hr || !aligned16(s->er) || !aligned16(s->ht) ||
+        !aligned16(s->et) || !aligned16(s->hr))
+      return fail(GEO_ERR_INVALID, "field buffers must be non-null and 16-byte aligned");
+    if (si_on && !s->si_memory)
+      return fail(GEO_ERR_INVALID, "surface impedance memory buffer is required");
+    if (plasma && (!s->plasma_current || s->plasma_pitch != layout.plasma_pitch))
+      return fail(GEO_ERR_INVALID, "plasma current buffer or pitch does not match the plan");
+    drop_graph();
+    hp.er = static_cast<const T*>(s->er);
+    hp.et = static_cast<const T*>(s->et);
+    hp.ht = static_cast<T*>(s->ht);
+    hp.hr = static_cast<T*>(s->hr);
+    hp.si_mem = static_cast<C*>(s->si_memory);
+    hp.pl.current = static_cast<C*>(s->plasma_current);
+    ep.er = static_cast<T*>(s->er);
+    ep.et = static_cast<T*>(s->et);
+    ep.ht = static_cast<const T*>(s->ht);
+    ep.hr = static_cast<const T*>(s->hr);
+    ep.pl.current = static_cast<C*>(s->plasma_current);
+    bound = true;
+    return GEO_OK;
+  }
+
+  int set_receivers(const int64_t* v, const int64_t* l, const double* w, int32_t n) override {
+    if (n < 0) return fail(GEO_ERR_INVALID, "receiver count must be non-negative");
+    std::vector<int32_t> vv(3 * std::max(n, 1), 0), ll(std::max(n, 1), 0);
+    std::vector<C> ww(3 * std::max(n, 1), 0);
+    for (int32_t i = 0; i < n; ++i) {
+      if (l[i] < 0 || l[i] > R) return fail(GEO_ERR_INVALID, "receiver layer out of range");
+      ll[i] = (int32_t)l[i];
+      for (int q = 0; q < 3; ++q) {
+        if (v[3 * i + q] < 0 || v[3 * i + q] >= V)
+          return fail(GEO_ERR_INVALID, "receiver vertex out of range");
+        vv[3 * i + q] = (int32_t)v[3 * i + q];
+        ww[3 * i + q] = (C)w[3 * i + q];
+      }
+    }
+    drop_graph();
+    if (rx_mem) (void)hipFree(rx_mem);
+    rx_mem = nullptr;
+    const size_t nb = std::max(n, 1);
+    const size_t bytes_v = 3 * nb * sizeof(int32_t), bytes_l = nb * sizeof(int32_t);
+    const size_t off_w = round_up((int64_t)(bytes_v + bytes_l), 16);
+    GEO_HIP(hipMalloc(&rx_mem, off_w + 3 * nb * sizeof(C)));
+    char* base = static_cast<char*>(rx_mem);
+    GEO_HIP(hipMemcpy(base, vv.data(), bytes_v, hipMemcpyHostToDevice));
+    GEO_HIP(hipMemcpy(base + bytes_v, ll.data(), bytes_l, hipMemcpyHostToDevice));
+    GEO_HIP(hipMemcpy(base + off_w, ww.data(), 3 * nb * sizeof(C), hipMemcpyHostToDevice));
+    int32_t* dv = reinterpret_cast<int32_t*>(base);
+    int32_t* dl = reinterpret_cast<int32_t*>(base + bytes_v);
+    C* dw = reinterpret_cast<C*>(base + off_w);
+    hp.rx_vertex = dv;
+    hp.rx_layer = dl;
+    hp.rx_w = dw;
+    hp.n_rx = n;
+    n_rx = n;
+    return GEO_OK;
+  }
+
+  void launch_h(hipStream_t s) {
+    if (plasma) {
+      if (magnetized) hipLaunchKernelGGL((geo::h_step_kernel<T, C, true, true>), dim3(h_grid), dim3(geo::kBlock), 0, s, hp);
+      else hipLaunchKernelGGL((geo::h_step_kernel<T, C, true, false>), dim3(h_grid), dim3(geo::kBlock), 0, s, hp);
+    } else {
+      hipLaunchKernelGGL((geo::h_step_kernel<T, C, false, false>), dim3(h_grid), dim3(geo::kBlock), 0, s, hp);
+    }
+  }
+  void launch_e(hipStream_t s) {
+    if (plasma) hipLaunchKernelGGL((geo::e_step_kernel<T, C, true>), dim3(e_grid), dim3(geo::kBlock), 0, s, ep);
+    else hipLaunchKernelGGL((geo::e_step_kernel<T, C, false>), dim3(e_grid), dim3(geo::kBlock), 0, s, ep);
+  }
+
+  int ensure_graph(int32_t chunk) {
+    if (graph_exec && graph_chunk == chunk) return GEO_OK;
+    drop_graph();
+    GEO_HIP(hipStreamBeginCapture(capture_stream, hipStreamCaptureModeThreadLocal));
+    for (int32_t i = 0; i < chunk; ++i) {
+      launch_h(capture_stream);
+      launch_e(capture_stream);
+    }
+    hipGraph_t g = nullptr;
+    GEO_HIP(hipStreamEndCapture(capture_stream, &g));
+    graph = g;
+    GEO_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    graph_chunk = chunk;
+    return GEO_OK;
+  }
+
+  int step(const void* currents, int64_t n, int32_t chunk, void* trace,
+           hipStream_t stream) override {
+    if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
+    if (n < 0) return fail(GEO_ERR_INVALID, "step count must be non-negative");
+    if (trace != nullptr && n_rx <= 0)
+      return fail(GEO_ERR_INVALID, "trace recording requires receivers");
+    hipLaunchKernelGGL(geo::set_dynamic_kernel, dim3(1), dim3(1), 0, stream, dyn, (int64_t)0,
+                       trace, trace ? n : (int64_t)0, currents);
+    GEO_HIP(hipGetLastError());
+    int64_t done = 0;
+    if (chunk > 1 && n >= chunk) {
+      int rc = ensure_graph(chunk);
+      if (rc != GEO_OK) return rc;
+      for (; done + chunk <= n; done += chunk) GEO_HIP(hipGraphLaunch(graph_exec, stream));
+    }
+    for (; done < n; ++done) {
+      launch_h(stream);
+      launch_e(stream);
+    }
+    GEO_HIP(hipGetLastError());
+    if (trace != nullptr) {
+      C* last = static_cast<C*>(trace) + n * n_rx;
+      return sample(last, stream);
+    }
+    return GEO_OK;
+  }
+
+  int sample(void* row, hipStream_t stream) override {
+    if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
+    if (n_rx <= 0) return fail(GEO_ERR_INVALID, "no receivers are set");
+    const int threads = 64;
+    hipLaunchKernelGGL((geo::sample_kernel<T, C>), dim3((n_rx + threads - 1) / threads),
+                       dim3(threads), 0, stream, hp.er, hp.pn, hp.s_er_inv, hp.rx_vertex,
+                       hp.rx_layer, hp.rx_w, n_rx, static_cast<C*>(row));
+    GEO_HIP(hipGetLastError());
+    return GEO_OK;
+  }
+
+  int diag_one(const T* x, int64_t rows, int32_t cols, int32_t pitch, C inv_scale,
+               const double* row_w, const double* col_w, const double* eps, int64_t eps_stride,
+               double coef, int slot, hipStream_t stream) {
+    DiagAcc* part = diag_partial + slot * kDiagBlocks;
+    hipLaunchKernelGGL((diag_field_kernel<T, C>), dim3(kDiagBlocks), dim3(geo::kBlock), 0,
+                       stream, x, rows, cols, pitch, inv_scale, row_w, col_w, eps, eps_stride,
+                       coef, part);
+    hipLaunchKernelGGL(diag_reduce_kernel, dim3(1), dim3(geo::kBlock), 0, stream, part,
+                       kDiagBlocks, diag_out + slot);
+    GEO_HIP(hipGetLastError());
+    return GEO_OK;
+  }
+
+  int diagnostics(GeoDiagOut* out, hipStream_t stream) override {
+    if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
+    int rc;
+    if ((rc = diag_one(hp.er, V, (int32_t)(R + 1), (int32_t)pn, hp.s_er_inv, w_er_row,
+                       w_er_col, eps_er_dev, eps_er_stride, 0.5, 0, stream)) != GEO_OK)
+      return rc;
+    if ((rc = diag_one(hp.et, E, (int32_t)R, (int32_t)pm, hp.s_et_inv, w_et_row, w_et_col,
+                       eps_et_dev, eps_et_stride, 0.5, 1, stream)) != GEO_OK)
+      return rc;
+    if ((rc = diag_one(hp.hr, F, (int32_t)R, (int32_t)pm, hp.s_hr_inv, w_hr_row, w_hr_col,
+                       nullptr, 0, 0.5 * kMu0, 2, stream)) != GEO_OK)
+      return rc;
+    if ((rc = diag_one(hp.ht, E, (int32_t)(R + 1), (int32_t)pn, hp.s_ht_inv, w_ht_row,
+                       w_ht_col, nullptr, 0, 0.5 * kMu0, 3, stream)) != GEO_OK)
+      return rc;
+    DiagAcc host[4];
+    GEO_HIP(hipMemcpyAsync(host, diag_out, sizeof(host), hipMemcpyDeviceToHost, stream));
+    GEO_HIP(hipStreamSynchronize(stream));
+    out->max_abs_er = host[0].max_abs;
+    out->max_abs_et = host[1].max_abs;
+    out->max_abs_hr = host[2].max_abs;
+    out->max_abs_ht = host[3].max_abs;
+    out->energy_er = host[0].energy;
+    out->energy_et = host[1].energy;
+    out->energy_hr = host[2].energy;
+    out->energy_ht = host[3].energy;
+    out->nonfinite = host[0].nonfinite + host[1].nonfinite + host[2].nonfinite +
+                     host[3].nonfinite;
+    return GEO_OK;
+  }
+
+  int time_kernels(int64_t n, hipStream_t stream, double* out_ms) override {
+    if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
+    if (n <= 0) return fail(GEO_ERR_INVALID, "step count must be positive");
+    hipLaunchKernelGGL(geo::set_dynamic_kernel, dim3(1), dim3(1), 0, stream, dyn, (int64_t)0,
+                       (void*)nullptr, (int64_t)0, (const void*)nullptr);
+    std::vector<hipEvent_t> ev(3 * n);
+    for (auto& e : ev) GEO_HIP(hipEventCreate(&e));
+    for (int64_t i = 0; i < n; ++i) {
+      GEO_HIP(hipEventRecord(ev[3 * i], stream));
+      launch_h(stream);
+      GEO_HIP(hipEventRecord(ev[3 * i + 1], stream));
+      launch_e(stream);
+      GEO_HIP(hipEventRecord(ev[3 * i + 2], stream));
+    }
+    GEO_HIP(hipGetLastError());
+    GEO_HIP(hipEventSynchronize(ev[3 * n - 1]));
+    double h = 0.0, e = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      float a = 0.f, b = 0.f;
+      GEO_HIP(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
+      GEO_HIP(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
+      h += a;
+      e += b;
+    }
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    out_ms[0] = h / n;
+    out_ms[1] = e / n;
+    return GEO_OK;
+  }
+};
+
+int check_device() {
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
+    return fail(GEO_ERR_UNAVAILABLE, "no HIP device is available");
+  int dev = 0;
+  GEO_HIP(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  GEO_HIP(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(GEO_ERR_UNAVAILABLE,
+                std::string("this library is built for gfx950; device is ") + prop.gcnArchName);
+  return GEO_OK;
+}
+
+}  // namespace
+
+struct GeoPlan {
+  std::unique_ptr<PlanBase> impl;
+};
+
+extern "C" {
+
+int geo_version(void) { return 1; }
+
+const char* geo_last_error(void) { return g_error.c_str(); }
+
+int geo_plan_create(const GeoMeshDesc* mesh, const GeoRadialDesc* radial,
+                    const GeoPhysicsDesc* physics, GeoPlan** out) {
+  if (!mesh || !radial || !physics || !out) return fail(GEO_ERR_INVALID, "null argument");
+  *out = nullptr;
+  int rc = check_device();
+  if (rc != GEO_OK) return rc;
+  std::unique_ptr<PlanBase> impl;
+  switch (physics->storage) {
+    case GEO_STORAGE_F32: {
+      auto p = std::make_unique<Plan<float, float>>();
+      rc = p->create(mesh, radial, physics);
+      impl = std::move(p);
+      break;
+    }
+    case GEO_STORAGE_F64: {
+      auto p = std::make_unique<Plan<double, double>>();
+      rc = p->create(mesh, radial, physics);
+      impl = std::move(p);
+      break;
+    }
+    case GEO_STORAGE_F16: {
+      auto p = std::make_unique<Plan<__half, float>>();
+      rc = p->create(mesh, radial, physics);
+      impl = std::move(p);
+      break;
+    }
+    case GEO_STORAGE_BF16: {
+      auto p = std::make_unique<Plan<__hip_bfloat16, float>>();
+      rc = p->create(mesh, radial, physics);
+      impl = std::move(p);
+      break;
+    }
+    default:
+      return fail(GEO_ERR_INVALID, "unknown storage type");
+  }
+  if (rc != GEO_OK) return rc;
+  *out = new GeoPlan{std::move(impl)};
+  return GEO_OK;
+}
+
+int geo_plan_layout(const GeoPlan* plan, GeoLayout* out) {
+  if (!plan || !out) return fail(GEO_ERR_INVALID, "null argument");
+  *out = plan->impl->layout;
+  return GEO_OK;
+}
+
+int geo_plan_bind(GeoPlan* plan, const GeoState* state) {
+  if (!plan || !state) return fail(GEO_ERR_INVALID, "null argument");
+  return plan->impl->bind(state);
+}
+
+int geo_step(GeoPlan* plan, const void* currents, int64_t n_steps, int32_t chunk, void* trace,
+             void* stream) {
+  if (!plan) return fail(GEO_ERR_INVALID, "null plan");
+  return plan->impl->step(currents, n_steps, chunk, trace, static_cast<hipStream_t>(stream));
+}
+
+int geo_set_receivers(GeoPlan* plan, const int64_t* vertices, const int64_t* layers,
+                      const double* weights, int32_t n_receivers) {
+  if (!plan) return fail(GEO_ERR_INVALID, "null plan");
+  if (n_receivers > 0 && (!vertices || !layers || !weights))
+    return fail(GEO_ERR_INVALID, "null receiver tables");
+  return plan->impl->set_receivers(vertices, layers, weights, n_receivers);
+}
+
+int geo_sample_er(GeoPlan* plan, void* row, void* stream) {
+  if (!plan || !row) return fail(GEO_ERR_INVALID, "null argument");
+  return plan->impl->sample(row, static_cast<hipStream_t>(stream));
+}
+
+int geo_diagnostics(GeoPlan* plan, GeoDiagOut* out, void* stream) {
+  if (!plan || !out) return fail(GEO_ERR_INVALID, "null argument");
+  return plan->impl->diagnostics(out, static_cast<hipStream_t>(stream));
+}
+
+int geo_time_kernels(GeoPlan* plan, int64_t n_steps, void* stream, double* out_ms) {
+  if (!plan || !out_ms) return fail(GEO_ERR_INVALID, "null argument");
+  return plan->impl->time_kernels(n_steps, static_cast<hipStream_t>(stream), out_ms);
+}
+
+void geo_plan_destroy(GeoPlan* plan) { delete plan; }
+
+}  // extern "C"

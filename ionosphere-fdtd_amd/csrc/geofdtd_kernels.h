@@ -1,0 +1,649 @@
+// Device code of the GeodesicFDTD step: two fused kernels per leapfrog step.
+//
+// Layout.  Every field is stored radial-innermost: one row per mesh entity
+// (vertex, edge or face) holding its radial column, rows padded to a multiple
+// of one 16-byte strip.  A thread owns one strip (4 FP32 / 2 FP64 / 8 FP16
+// values) of one column, so the own-column traffic is 16-byte vector loads
+// and stores, and each unstructured neighbour gather (the 2 Er columns of an
+// edge, 3 Et columns of a face, <= 6 Ht columns of a dual cell, 2 Hr columns
+// of an edge) reads a contiguous column slab that the strips of one wave
+// cover with coalesced 16-byte requests.
+//
+// Kernels.  h_step_kernel advances Ht and Hr (reference solver.py:877-921),
+// the surface-impedance lower boundary (surface_impedance.py:225-270) and the
+// plasma ADE current (plasma.py:413-442, 489-500) which reads the same Et
+// columns as the Hr curl; one block of it samples receivers
+// (solver.py:1122-1124).  e_step_kernel advances Er and Et
+// (solver.py:923-989) including the plasma current scatter
+// (plasma.py:503-527) and the source terms (solver.py:936-959, 973-987).
+// Blocks of the different segments are interleaved by a host-built block map
+// and remapped so that each XCD sweeps one contiguous range of the mesh.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+#include <stdint.h>
+
+namespace geo {
+
+constexpr int kBlock = 256;
+constexpr int kMaxSource = 16;
+constexpr int kMaxSiTerms = 32;
+constexpr int kMaxSpecies = 4;
+constexpr int kNumXcd = 8;
+
+enum Segment : uint32_t {
+  kSegHEdge = 0, kSegHFace = 1, kSegSi = 2, kSegSample = 3,
+  kSegEVertex = 4, kSegEEdge = 5,
+};
+
+// Column segment geometry: an integer number of columns per 256-thread
+// block, column index within the block by a 16-bit reciprocal multiply
+// (exact for tid < 256 and strips <= 256).
+struct SegGeom {
+  int32_t n_rows;
+  int32_t strips;         // 16-byte strips per row
+  int32_t rows_per_block;
+  uint32_t magic;         // ceil(65536 / strips)
+};
+
+struct Dynamic {
+  int64_t counter;        // steps completed since geo_step started
+  int64_t trace_rows;     // rows in the trace buffer (0: no trace)
+  void* trace;            // (trace_rows, n_receivers) compute type
+  const void* currents;   // (n,) compute type, or nullptr
+};
+
+template <typename T> struct Strip {
+  static constexpr int N = 16 / sizeof(T);
+  T v[N];
+} __attribute__((aligned(16)));
+
+template <typename T> __device__ __forceinline__ constexpr bool is_narrow() {
+  return sizeof(T) == 2;
+}
+
+template <typename C> __device__ __forceinline__ C to_c(float x) { return (C)x; }
+template <typename C> __device__ __forceinline__ C to_c(double x) { return (C)x; }
+template <typename C> __device__ __forceinline__ C to_c(__half x) { return (C)__half2float(x); }
+template <typename C> __device__ __forceinline__ C to_c(__hip_bfloat16 x) {
+  return (C)__bfloat162float(x);
+}
+template <typename T> __device__ __forceinline__ T from_c(float x);
+template <> __device__ __forceinline__ float from_c<float>(float x) { return x; }
+template <> __device__ __forceinline__ __half from_c<__half>(float x) { return __float2half(x); }
+template <> __device__ __forceinline__ __hip_bfloat16 from_c<__hip_bfloat16>(float x) {
+  return __float2bfloat16(x);
+}
+template <typename T> __device__ __forceinline__ T from_c(double x);
+template <> __device__ __forceinline__ double from_c<double>(double x) { return x; }
+
+// Load one strip of N values and convert to compute type (undoing the
+// power-of-two storage scale for narrow types).
+template <typename T, typename C>
+__device__ __forceinline__ void load_strip(const T* __restrict__ p, C* out, C inv_scale) {
+  constexpr int N = Strip<T>::N;
+  Strip<T> s = *reinterpret_cast<const Strip<T>*>(p);
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if constexpr (is_narrow<T>()) out[i] = to_c<C>(s.v[i]) * inv_scale;
+    else out[i] = to_c<C>(s.v[i]);
+  }
+}
+
+template <typename T, typename C>
+__device__ __forceinline__ void store_strip(T* __restrict__ p, const C* in, C scale) {
+  constexpr int N = Strip<T>::N;
+  Strip<T> s;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    if constexpr (is_narrow<T>()) s.v[i] = from_c<T>(in[i] * scale);
+    else s.v[i] = from_c<T>(in[i]);
+  }
+  *reinterpret_cast<Strip<T>*>(p) = s;
+}
+
+template <typename T, typename C>
+__device__ __forceinline__ C load_one(const T* __restrict__ p, C inv_scale) {
+  if constexpr (is_narrow<T>()) return to_c<C>(*p) * inv_scale;
+  else return to_c<C>(*p);
+}
+
+template <typename T, typename C>
+__device__ __forceinline__ void store_one(T* __restrict__ p, C x, C scale) {
+  if constexpr (is_narrow<T>()) *p = from_c<T>(x * scale);
+  else *p = from_c<T>(x);
+}
+
+// Logical block index: hardware blocks are dispatched round-robin over the 8
+// XCDs, so block b runs on XCD b % 8.  Give XCD x the contiguous logical range
+// [x * per_xcd, (x + 1) * per_xcd) so neighbour columns stay in its L2.
+__device__ __forceinline__ int32_t logical_block(int32_t per_xcd) {
+  const int32_t b = blockIdx.x;
+  return (b % kNumXcd) * per_xcd + b / kNumXcd;
+}
+
+__device__ __forceinline__ void row_and_strip(const SegGeom& g, int32_t block, int32_t* row,
+                                              int32_t* strip) {
+  const uint32_t tid = threadIdx.x;
+  const uint32_t local_row = (tid * g.magic) >> 16;
+  *strip = (int32_t)(tid - local_row * (uint32_t)g.strips);
+  *row = (local_row < (uint32_t)g.rows_per_block)
+             ? block * g.rows_per_block + (int32_t)local_row
+             : 0x7fffffff;
+}
+
+template <typename C>
+struct PlasmaTables {
+  int32_t n_species;
+  int32_t band_lo, band_hi;     // band_lo is a multiple of the strip width
+  int32_t pitch;                // J row pitch (compute elements)
+  int64_t species_stride;       // elements between species blocks (3 * F * pitch)
+  int64_t comp_stride;          // elements between x/y/z blocks (F * pitch)
+  C* current;                   // (species, 3, F, pitch)
+  const C* recon;               // (F, 9)
+  const C* center;              // (F, 3)
+  const C* left_tan;            // (E, 3)
+  const C* right_tan;           // (E, 3)
+  const int32_t* vertex_faces;  // (V, 6)
+  const C* vertex_face_w;       // (V, 6)
+  const C* coef;                // (species, 6, rows, pitch)
+  int64_t coef_row_stride;      // 0 (uniform profile) or pitch
+  int64_t coef_q_stride;        // elements between coefficient kinds
+  int64_t coef_species_stride;
+  const C* bdir;                // (rows, pitch, 3) unit B, magnetized only
+  int64_t b_row_stride;
+};
+
+template <typename T, typename C>
+struct HParams {
+  const T* __restrict__ er;
+  const T* __restrict__ et;
+  T* __restrict__ ht;
+  T* __restrict__ hr;
+  int32_t pn, pm;               // node / midpoint row pitch
+  int32_t R;
+  int32_t si_on;
+  C s_er_inv, s_et_inv, s_ht, s_ht_inv, s_hr, s_hr_inv;
+  const int32_t* __restrict__ edges;      // (E,2)
+  const C* __restrict__ edge_grad;        // (E,) dt/(mu0 theta)
+  const int32_t* __restrict__ face_edges; // (F,3)
+  const int32_t* __restrict__ faces;      // (F,3)
+  const C* __restrict__ face_curl;        // (F,3) dt/mu0 s theta / Omega
+  const C* __restrict__ inv_r;            // (pn,)
+  const C* __restrict__ rd_b;             // (pn,) dt/mu0 alpha_k
+  const C* __restrict__ rd_c;             // (pn,) dt/mu0 gamma_k
+  const C* __restrict__ inv_rbar;         // (pm,)
+  // surface impedance
+  C* __restrict__ si_mem;
+  const C* __restrict__ si_scale;         // (rows,) sqrt(mu/sigma)
+  int32_t si_scale_stride;                // 0 or 1
+  int32_t si_terms;
+  C si_cs;                                // 2 dt/(mu0 dr0)
+  C si_cs_metric;                         // si_cs * rbar0/r0
+  C si_hw_sum;                            // sum of history weights
+  C si_decay[kMaxSiTerms], si_drive[kMaxSiTerms], si_hw[kMaxSiTerms];
+  PlasmaTables<C> pl;
+  // receivers + step counter
+  Dynamic* dyn;
+  const int32_t* __restrict__ rx_vertex;  // (n,3)
+  const int32_t* __restrict__ rx_layer;   // (n,)
+  const C* __restrict__ rx_w;             // (n,3)
+  int32_t n_rx;
+  // grid
+  const uint32_t* __restrict__ block_map;
+  int32_t n_blocks, per_xcd;
+  SegGeom g_edge, g_face;
+  int32_t n_edges;
+};
+
+template <typename T, typename C>
+struct EParams {
+  T* __restrict__ er;
+  T* __restrict__ et;
+  const T* __restrict__ ht;
+  const T* __restrict__ hr;
+  int32_t pn, pm, R;
+  int32_t ca_er_stride, ca_et_stride;     // 0 or row pitch
+  C s_er, s_er_inv, s_et, s_et_inv, s_ht_inv, s_hr_inv;
+  const int32_t* __restrict__ vertex_edges; // (V,6)
+  const C* __restrict__ vertex_w;           // (V,6) sign*phi/A
+  const int32_t* __restrict__ edge_lr;      // (E,2) left, right face
+  const C* __restrict__ edge_dual;          // (E,) 1/phi
+  const C* __restrict__ ca_er;
+  const C* __restrict__ cb_er;
+  const C* __restrict__ ca_et;
+  const C* __restrict__ cb_et;
+  const C* __restrict__ inv_r;            // (pn,)
+  const C* __restrict__ inv_rbar;         // (pm,)
+  const C* __restrict__ rh_a;             // (pm,) alpha'_j
+  const C* __restrict__ rh_c;             // (pm,) gamma'_j
+  PlasmaTables<C> pl;
+  // sources: vertical (Er) and tangential (Et)
+  int32_t n_src, n_tsrc;
+  int32_t src_row[kMaxSource], src_layer[kMaxSource];
+  C src_factor[kMaxSource];
+  int32_t tsrc_row[kMaxSource], tsrc_layer[kMaxSource];
+  C tsrc_factor[kMaxSource];
+  Dynamic* dyn;
+  const uint32_t* __restrict__ block_map;
+  int32_t n_blocks, per_xcd;
+  SegGeom g_vertex, g_edge;
+};
+
+// ---------------------------------------------------------------------------
+// H half-step
+// ---------------------------------------------------------------------------
+
+template <typename T, typename C>
+__device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, int32_t e, int32_t strip) {
+  constexpr int N = Strip<T>::N;
+  const int32_t k0 = strip * N;
+  const int32_t tail = p.edges[2 * e];
+  const int32_t head = p.edges[2 * e + 1];
+  const C a = p.edge_grad[e];
+  C ert[N], erh[N], ht[N], et[N];
+  load_strip(p.er + (size_t)tail * p.pn + k0, ert, p.s_er_inv);
+  load_strip(p.er + (size_t)head * p.pn + k0, erh, p.s_er_inv);
+  T* hrow = p.ht + (size_t)e * p.pn + k0;
+  load_strip(hrow, ht, p.s_ht_inv);
+  const T* erow = p.et + (size_t)e * p.pm;
+  if (k0 < p.pm) {
+    load_strip(erow + k0, et, p.s_et_inv);
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) et[i] = (C)0;
+  }
+  C prev = (k0 > 0) ? load_one(erow + k0 - 1, p.s_et_inv) : (C)0;
+  // Rows are padded with zeros and the radial tables are zero beyond R, so
+  // the strip is updated unconditionally: Et(R) reads the zero padding (odd
+  // ghost, PEC cap, solver.py:448-449) and padding lanes stay unchanged.
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int32_t k = k0 + i;
+    const C cur = et[i];
+    const C rd = p.rd_b[k] * (cur - prev) + p.rd_c[k] * cur;
+    ht[i] = ht[i] + a * p.inv_r[k] * (erh[i] - ert[i]) - rd;
+    prev = cur;
+  }
+  if (p.si_on && k0 == 0) {
+    // element 0 belongs to the surface-impedance segment
+#pragma unroll
+    for (int i = 1; i < N; ++i) store_one(hrow + i, ht[i], p.s_ht);
+  } else {
+    store_strip(hrow, ht, p.s_ht);
+  }
+}
+
+template <typename C>
+__device__ __forceinline__ void cross_b(const C* v, const C* b, C* out) {
+  out[0] = v[1] * b[2] - v[2] * b[1];
+  out[1] = v[2] * b[0] - v[0] * b[2];
+  out[2] = v[0] * b[1] - v[1] * b[0];
+}
+
+template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
+__device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, int32_t f, int32_t strip) {
+  constexpr int N = Strip<T>::N;
+  const int32_t j0 = strip * N;
+  const int32_t e0 = p.face_edges[3 * f];
+  const int32_t e1 = p.face_edges[3 * f + 1];
+  const int32_t e2 = p.face_edges[3 * f + 2];
+  const C w0 = p.face_curl[3 * f], w1 = p.face_curl[3 * f + 1], w2 = p.face_curl[3 * f + 2];
+  C et0[N], et1[N], et2[N], hr[N];
+  load_strip(p.et + (size_t)e0 * p.pm + j0, et0, p.s_et_inv);
+  load_strip(p.et + (size_t)e1 * p.pm + j0, et1, p.s_et_inv);
+  load_strip(p.et + (size_t)e2 * p.pm + j0, et2, p.s_et_inv);
+  T* hrow = p.hr + (size_t)f * p.pm + j0;
+  load_strip(hrow, hr, p.s_hr_inv);
+#pragma unroll
+  for (int i = 0; i < N; ++i)  // inv_rbar is zero on padding lanes
+    hr[i] = hr[i] - p.inv_rbar[j0 + i] * (w0 * et0[i] + w1 * et1[i] + w2 * et2[i]);
+  store_strip(hrow, hr, p.s_hr);
+
+  if constexpr (PLASMA) {
+    const PlasmaTables<C>& pl = p.pl;
+    if (j0 + N <= pl.band_lo || j0 >= pl.band_hi) return;
+    // E at (face, midpoint): tangential reconstruction + radial node average
+    // (plasma.py:492-500).
+    const int32_t v0 = p.faces[3 * f], v1 = p.faces[3 * f + 1], v2 = p.faces[3 * f + 2];
+    C a0[N + 1], a1[N + 1], a2[N + 1];
+    load_strip(p.er + (size_t)v0 * p.pn + j0, a0, p.s_er_inv);
+    load_strip(p.er + (size_t)v1 * p.pn + j0, a1, p.s_er_inv);
+    load_strip(p.er + (size_t)v2 * p.pn + j0, a2, p.s_er_inv);
+    // node j0+N is needed only while midpoint j0+N-1 < R, i.e. j0+N <= R
+    const bool top = j0 + N <= p.R;
+    a0[N] = top ? load_one(p.er + (size_t)v0 * p.pn + j0 + N, p.s_er_inv) : (C)0;
+    a1[N] = top ? load_one(p.er + (size_t)v1 * p.pn + j0 + N, p.s_er_inv) : (C)0;
+    a2[N] = top ? load_one(p.er + (size_t)v2 * p.pn + j0 + N, p.s_er_inv) : (C)0;
+    C rc[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) rc[q] = pl.recon[9 * f + q];
+    const C n0 = pl.center[3 * f], n1 = pl.center[3 * f + 1], n2 = pl.center[3 * f + 2];
+    const int32_t jb0 = j0 - pl.band_lo;
+    for (int s = 0; s < pl.n_species; ++s) {
+      C* jx = pl.current + s * pl.species_stride + (size_t)f * pl.pitch + jb0;
+      C* jy = jx + pl.comp_stride;
+      C* jz = jy + pl.comp_stride;
+      const C* cf = pl.coef + s * pl.coef_species_stride + (size_t)f * pl.coef_row_stride + jb0;
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int32_t j = j0 + i;
+        if (j < pl.band_lo || j >= pl.band_hi) continue;
+        const C lower = (a0[i] + a1[i] + a2[i]) / (C)3;
+        const C upper = (a0[i + 1] + a1[i + 1] + a2[i + 1]) / (C)3;
+        const C rad = (C)0.5 * (lower + upper);
+        C E[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+          E[d] = (et0[i] * rc[d] + et1[i] * rc[3 + d]) + et2[i] * rc[6 + d];
+        E[0] += rad * n0;
+        E[1] += rad * n1;
+        E[2] += rad * n2;
+        C J[3] = {jx[i], jy[i], jz[i]};
+        const C decay = cf[i];
+        C out[3];
+        if constexpr (!MAGNETIZED) {
+          const C dre = cf[4 * pl.coef_q_stride + i];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) out[d] = decay * J[d] + dre * E[d];
+        } else {
+          const C cs = cf[pl.coef_q_stride + i];
+          const C sn = cf[2 * pl.coef_q_stride + i];
+          const C dpar = cf[3 * pl.coef_q_stride + i];
+          const C dre = cf[4 * pl.coef_q_stride + i];
+          const C dim = cf[5 * pl.coef_q_stride + i];
+          const C* bp = pl.bdir + (size_t)f * pl.b_row_stride + 3 * (jb0 + i);
+          const C b[3] = {bp[0], bp[1], bp[2]};
+          const C es = (E[0] * b[0] + E[1] * b[1]) + E[2] * b[2];
+          const C js = (J[0] * b[0] + J[1] * b[1]) + J[2] * b[2];
+          C epar[3], eperp[3], jpar[3], jperp[3], ecr[3], jcr[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            epar[d] = es * b[d];
+            eperp[d] = E[d] - epar[d];
+            jpar[d] = js * b[d];
+            jperp[d] = J[d] - jpar[d];
+          }
+          cross_b(eperp, b, ecr);
+          cross_b(jperp, b, jcr);
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+            out[d] = decay * jpar[d] + dpar * epar[d] + decay * (cs * jperp[d] + sn * jcr[d]) +
+                     dre * eperp[d] + dim * ecr[d];
+        }
+        jx[i] = out[0];
+        jy[i] = out[1];
+        jz[i] = out[2];
+      }
+    }
+  }
+}
+
+template <typename T, typename C>
+__device__ __forceinline__ void h_si_edge(const HParams<T, C>& p, int32_t e) {
+  // Implicit lower-boundary Ht with 16-pole diffusive memory
+  // (surface_impedance.py:225-270).
+  const int32_t tail = p.edges[2 * e];
+  const int32_t head = p.edges[2 * e + 1];
+  const C forcing = p.edge_grad[e] * p.inv_r[0] *
+                    (load_one(p.er + (size_t)head * p.pn, p.s_er_inv) -
+                     load_one(p.er + (size_t)tail * p.pn, p.s_er_inv));
+  T* hp = p.ht + (size_t)e * p.pn;
+  const C h_old = load_one(hp, p.s_ht_inv);
+  const C et0 = load_one(p.et + (size_t)e * p.pm, p.s_et_inv);
+  const C scale = p.si_scale[e * p.si_scale_stride];
+  C* mem = p.si_mem + (size_t)e * p.si_terms;
+  C hist = (C)0;
+  for (int i = 0; i < p.si_terms; ++i) hist += mem[i] * p.si_hw[i];
+  hist *= scale;
+  const C coupling = p.si_cs * (scale * p.si_hw_sum);
+  const C h_new = ((C)1 - (C)0.5 * coupling) * h_old + forcing - p.si_cs_metric * et0 +
+                  p.si_cs * hist;
+  const C h = h_new / ((C)1 + (C)0.5 * coupling);
+  store_one(hp, h, p.s_ht);
+  const C sum = h + h_old;
+  for (int i = 0; i < p.si_terms; ++i) mem[i] = mem[i] * p.si_decay[i] + sum * p.si_drive[i];
+}
+
+template <typename T, typename C>
+__device__ __forceinline__ void sample_block(const HParams<T, C>& p) {
+  // Receiver row for the Er state entering this step, then advance the
+  // device step counter (only this block reads or writes it in the H kernel).
+  __shared__ int64_t row;
+  if (threadIdx.x == 0) row = p.dyn->counter;
+  __syncthreads();
+  const int64_t r = row;
+  C* trace = reinterpret_cast<C*>(p.dyn->trace);
+  if (trace != nullptr && r < p.dyn->trace_rows) {
+    for (int i = threadIdx.x; i < p.n_rx; i += blockDim.x) {
+      const int32_t layer = p.rx_layer[i];
+      C acc = (C)0;
+#pragma unroll
+      for (int q = 0; q < 3; ++q)
+        acc += load_one(p.er + (size_t)p.rx_vertex[3 * i + q] * p.pn + layer, p.s_er_inv) *
+               p.rx_w[3 * i + q];
+      trace[r * p.n_rx + i] = acc;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) p.dyn->counter = r + 1;
+}
+
+template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
+__global__ void __launch_bounds__(kBlock) h_step_kernel(const HParams<T, C> p) {
+  const int32_t L = logical_block(p.per_xcd);
+  if (L >= p.n_blocks) return;
+  const uint32_t m = p.block_map[L];
+  const uint32_t seg = m >> 24;
+  const int32_t b = (int32_t)(m & 0xffffffu);
+  if (seg == kSegHEdge) {
+    int32_t row, strip;
+    row_and_strip(p.g_edge, b, &row, &strip);
+    if (row < p.g_edge.n_rows) h_edge_strip(p, row, strip);
+  } else if (seg == kSegHFace) {
+    int32_t row, strip;
+    row_and_strip(p.g_face, b, &row, &strip);
+    if (row < p.g_face.n_rows) h_face_strip<T, C, PLASMA, MAGNETIZED>(p, row, strip);
+  } else if (seg == kSegSi) {
+    const int32_t e = b * kBlock + (int32_t)threadIdx.x;
+    if (e < p.n_edges) h_si_edge(p, e);
+  } else {
+    sample_block(p);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// E half-step
+// ---------------------------------------------------------------------------
+
+// Sum over species of J(face, j) . vec, reference accumulation order.
+template <typename C>
+__device__ __forceinline__ C plasma_dot(const PlasmaTables<C>& pl, int32_t f, int32_t j,
+                                        C v0, C v1, C v2) {
+  const int32_t jb = j - pl.band_lo;
+  C jt[3] = {(C)0, (C)0, (C)0};
+  for (int s = 0; s < pl.n_species; ++s) {
+    const C* base = pl.current + s * pl.species_stride + (size_t)f * pl.pitch + jb;
+    jt[0] += base[0];
+    jt[1] += base[pl.comp_stride];
+    jt[2] += base[2 * pl.comp_stride];
+  }
+  return (jt[0] * v0 + jt[1] * v1) + jt[2] * v2;
+}
+
+template <typename T, typename C, bool PLASMA>
+__device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, int32_t v, int32_t strip) {
+  constexpr int N = Strip<T>::N;
+  const int32_t k0 = strip * N;
+  C curl[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) curl[i] = (C)0;
+#pragma unroll
+  for (int s = 0; s < 6; ++s) {
+    const C w = p.vertex_w[6 * v + s];  // 0 on degree-5 padding slots
+    const int32_t e = p.vertex_edges[6 * v + s];
+    C h[N];
+    load_strip(p.ht + (size_t)e * p.pn + k0, h, p.s_ht_inv);
+#pragma unroll
+    for (int i = 0; i < N; ++i) curl[i] += w * h[i];
+  }
+  T* erow = p.er + (size_t)v * p.pn + k0;
+  C er[N];
+  load_strip(erow, er, p.s_er_inv);
+  const C* ca = p.ca_er + (size_t)v * p.ca_er_stride;
+  const C* cb = p.cb_er + (size_t)v * p.ca_er_stride;
+  C jr[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) jr[i] = (C)0;
+  if constexpr (PLASMA) {
+    const PlasmaTables<C>& pl = p.pl;
+    // radial current at nodes k0-1 .. k0+N-1 needs midpoint layers k0-1 .. k0+N-1
+    if (k0 + N >= pl.band_lo && k0 - 1 < pl.band_hi) {
+      C rv[N + 1];
+#pragma unroll
+      for (int i = 0; i <= N; ++i) rv[i] = (C)0;
+      for (int s = 0; s < 6; ++s) {
+        const C w = pl.vertex_face_w[6 * v + s];
+        if (w == (C)0) continue;
+        const int32_t f = pl.vertex_faces[6 * v + s];
+        const C c0 = pl.center[3 * f], c1 = pl.center[3 * f + 1], c2 = pl.center[3 * f + 2];
+#pragma unroll
+        for (int i = 0; i <= N; ++i) {
+          const int32_t j = k0 - 1 + i;
+          if (j >= pl.band_lo && j < pl.band_hi) rv[i] += plasma_dot(pl, f, j, c0, c1, c2) * w;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int32_t k = k0 + i;
+        if (k == 0) jr[i] = rv[1];
+        else if (k == p.R) jr[i] = rv[i];
+        else jr[i] = (C)0.5 * (rv[i] + rv[i + 1]);
+      }
+    }
+  }
+  C src[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) src[i] = (C)0;
+  for (int q = 0; q < p.n_src; ++q) {
+    const int32_t i = p.src_layer[q] - k0;
+    if (p.src_row[q] == v && i >= 0 && i < N) {
+      const C* cur = reinterpret_cast<const C*>(p.dyn->currents);
+      const C value = p.src_factor[q] * (cur != nullptr ? cur[p.dyn->counter - 1] : (C)0);
+#pragma unroll
+      for (int t = 0; t < N; ++t)
+        if (t == i) src[t] += value;
+    }
+  }
+  // ca/cb are zero on padding lanes, which therefore stay exactly zero.
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int32_t k = k0 + i;
+    er[i] = ca[k] * er[i] + cb[k] * (p.inv_r[k] * curl[i] - jr[i] - src[i]);
+  }
+  store_strip(erow, er, p.s_er);
+}
+
+template <typename T, typename C, bool PLASMA>
+__device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, int32_t e, int32_t strip) {
+  constexpr int N = Strip<T>::N;
+  const int32_t j0 = strip * N;
+  const int32_t fl = p.edge_lr[2 * e];
+  const int32_t fr = p.edge_lr[2 * e + 1];
+  const C inv_phi = p.edge_dual[e];
+  C hl[N], hrr[N], ht[N + 1], et[N];
+  load_strip(p.hr + (size_t)fl * p.pm + j0, hl, p.s_hr_inv);
+  load_strip(p.hr + (size_t)fr * p.pm + j0, hrr, p.s_hr_inv);
+  const T* hrow = p.ht + (size_t)e * p.pn;
+  load_strip(hrow + j0, ht, p.s_ht_inv);
+  ht[N] = (j0 + N <= p.R) ? load_one(hrow + j0 + N, p.s_ht_inv) : (C)0;
+  T* erow = p.et + (size_t)e * p.pm + j0;
+  load_strip(erow, et, p.s_et_inv);
+  const C* ca = p.ca_et + (size_t)e * p.ca_et_stride;
+  const C* cb = p.cb_et + (size_t)e * p.ca_et_stride;
+  C jt[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) jt[i] = (C)0;
+  if constexpr (PLASMA) {
+    const PlasmaTables<C>& pl = p.pl;
+    if (j0 + N > pl.band_lo && j0 < pl.band_hi) {
+      const C l0 = pl.left_tan[3 * e], l1 = pl.left_tan[3 * e + 1], l2 = pl.left_tan[3 * e + 2];
+      const C r0 = pl.right_tan[3 * e], r1 = pl.right_tan[3 * e + 1], r2 = pl.right_tan[3 * e + 2];
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int32_t j = j0 + i;
+        if (j >= pl.band_lo && j < pl.band_hi)
+          jt[i] = (C)0.5 * (plasma_dot(pl, fl, j, l0, l1, l2) + plasma_dot(pl, fr, j, r0, r1, r2));
+      }
+    }
+  }
+  C src[N];
+#pragma unroll
+  for (int i = 0; i < N; ++i) src[i] = (C)0;
+  for (int q = 0; q < p.n_tsrc; ++q) {
+    const int32_t i = p.tsrc_layer[q] - j0;
+    if (p.tsrc_row[q] == e && i >= 0 && i < N) {
+      const C* cur = reinterpret_cast<const C*>(p.dyn->currents);
+      const C value = p.tsrc_factor[q] * (cur != nullptr ? cur[p.dyn->counter - 1] : (C)0);
+#pragma unroll
+      for (int t = 0; t < N; ++t)
+        if (t == i) src[t] += value;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const int32_t j = j0 + i;
+    const C grad = p.inv_rbar[j] * inv_phi * (hl[i] - hrr[i]);
+    const C rd = p.rh_a[j] * (ht[i + 1] - ht[i]) + p.rh_c[j] * ht[i + 1];
+    et[i] = ca[j] * et[i] + cb[j] * (grad - rd - jt[i] - src[i]);
+  }
+  store_strip(erow, et, p.s_et);
+}
+
+template <typename T, typename C, bool PLASMA>
+__global__ void __launch_bounds__(kBlock) e_step_kernel(const EParams<T, C> p) {
+  const int32_t L = logical_block(p.per_xcd);
+  if (L >= p.n_blocks) return;
+  const uint32_t m = p.block_map[L];
+  const uint32_t seg = m >> 24;
+  const int32_t b = (int32_t)(m & 0xffffffu);
+  int32_t row, strip;
+  if (seg == kSegEVertex) {
+    row_and_strip(p.g_vertex, b, &row, &strip);
+    if (row < p.g_vertex.n_rows) e_vertex_strip<T, C, PLASMA>(p, row, strip);
+  } else {
+    row_and_strip(p.g_edge, b, &row, &strip);
+    if (row < p.g_edge.n_rows) e_edge_strip<T, C, PLASMA>(p, row, strip);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small helper kernels
+// ---------------------------------------------------------------------------
+
+__global__ void set_dynamic_kernel(Dynamic* dyn, int64_t counter, void* trace, int64_t rows,
+                                   const void* currents) {
+  dyn->counter = counter;
+  dyn->trace = trace;
+  dyn->trace_rows = rows;
+  dyn->currents = currents;
+}
+
+template <typename T, typename C>
+__global__ void sample_kernel(const T* __restrict__ er, int32_t pn, C s_er_inv,
+                              const int32_t* __restrict__ rx_vertex,
+                              const int32_t* __restrict__ rx_layer, const C* __restrict__ rx_w,
+                              int32_t n_rx, C* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_rx) return;
+  const int32_t layer = rx_layer[i];
+  C acc = (C)0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q)
+    acc += load_one(er + (size_t)rx_vertex[3 * i + q] * pn + layer, s_er_inv) * rx_w[3 * i + q];
+  out[i] = acc;
+}
+
+}  // namespace geo

@@ -1,0 +1,866 @@
+"""GeodesicFDTD on MI355X: the reference's solver API over the HIP step library.
+
+``SimulationConfig`` and ``GeodesicFDTD`` keep the reference's constructor,
+validation, attributes and methods (solver.py:30-1294).  Setup stays on the
+host in FP64 (mesh, radial grid, material sampling, CFL, exponential loss
+coefficients, source / receiver distributions, surface-impedance and plasma
+tables); ``libgeofdtd`` turns those tables into device tables once, and every
+``step`` runs the two fused HIP kernels per leapfrog step -- as a captured HIP
+graph of ``compile_chunk_size`` steps when ``compile_step=True``, otherwise
+as individual launches of the same kernels.
+
+Fields live in padded radial-innermost device buffers; ``er``, ``et``, ``hr``
+and ``ht`` are (entity, radial) views of them for FP32/FP64 storage, so
+in-place edits such as ``sim.er[:, k] = mode`` reach the kernels.  With
+FP16/BF16 storage the properties return unscaled FP32 copies and assignment
+goes through the setters.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Any
+
+import numpy as np
+from numpy.typing import NDArray
+
+from . import _native
+from .backends import ArrayBackend, BackendUnavailableError, create_backend
+from .constants import C_0, EARTH_RADIUS_M, EPSILON_0, MU_0
+from .materials import EarthIonosphereMaterial
+from .mesh import GeodesicMesh, build_geodesic_mesh, vertex_incidence
+from .plasma import MeshPlasmaModel, plasma_tables
+from .radial_grid import validate_radial_grid
+from .sources import GaussianCurrent, TangentialGaussianCurrent
+from .surface_impedance import ConductiveHalfSpaceSurface, surface_ade_tables
+
+DEFAULT_NARROW_SCALES = {"er": 1.0, "et": 1.0, "hr": 256.0, "ht": 256.0}
+
+
+@dataclass(frozen=True, slots=True)
+class SimulationConfig:
+    """Geometry and integration controls (reference solver.py:30-171)."""
+
+    subdivision: int = 2
+    radial_cells: int = 24
+    minimum_altitude_m: float = -100_000.0
+    maximum_altitude_m: float = 100_000.0
+    earth_radius_m: float = EARTH_RADIUS_M
+    courant_factor: float = 0.35
+    time_step_s: float | None = None
+    mesh_relaxations: int = 0
+    mesh_optimization_steps: int = 0
+    mesh_orientation: str = "polar"
+    radial_altitudes_m: tuple[float, ...] | None = None
+    radial_material_support: str = "point"
+    tangential_material_support: str = "point"
+    horizontal_anomaly_mode: str = "point"
+    radial_boundary_condition: str = "pec"
+    loss_integration: str = "exponential"
+    radial_grid_policy: str = "smooth"
+    geometry_mode: str = "full-spherical"
+    compress_uniform_material_coefficients: bool = False
+
+    def __post_init__(self) -> None:
+        ints = (self.subdivision, self.radial_cells, self.mesh_relaxations,
+                self.mesh_optimization_steps)
+        if any(isinstance(v, bool) or not isinstance(v, (int, np.integer)) for v in ints):
+            raise ValueError("mesh and radial cell controls must be integers")
+        if self.subdivision < 0:
+            raise ValueError("subdivision must be non-negative")
+        if self.radial_cells < 2:
+            raise ValueError("radial_cells must be at least 2")
+        if self.mesh_relaxations < 0:
+            raise ValueError("mesh_relaxations must be non-negative")
+        if self.mesh_optimization_steps < 0:
+            raise ValueError("mesh_optimization_steps must be non-negative")
+        if self.mesh_orientation not in {"native", "polar"}:
+            raise ValueError("mesh_orientation must be 'native' or 'polar'")
+        if self.radial_material_support not in {"point", "dual-cell"}:
+            raise ValueError("radial_material_support must be 'point' or 'dual-cell'")
+        if self.tangential_material_support not in {"point", "edge-diamond"}:
+            raise ValueError("tangential_material_support must be 'point' or 'edge-diamond'")
+        if self.horizontal_anomaly_mode not in {"point", "conservative-nearest"}:
+            raise ValueError("horizontal_anomaly_mode must be 'point' or 'conservative-nearest'")
+        if self.radial_boundary_condition not in {"pec", "surface-impedance"}:
+            raise ValueError("radial_boundary_condition must be 'pec' or 'surface-impedance'")
+        if self.radial_boundary_condition == "surface-impedance" and \
+                self.minimum_altitude_m != 0.0:
+            raise ValueError("surface-impedance boundary requires minimum_altitude_m=0")
+        if self.loss_integration not in {"exponential", "trapezoidal"}:
+            raise ValueError("loss_integration must be 'exponential' or 'trapezoidal'")
+        if self.radial_grid_policy not in {"smooth", "balanced-2to1", "allow-abrupt"}:
+            raise ValueError("radial_grid_policy must be 'smooth', 'balanced-2to1', "
+                             "or 'allow-abrupt'")
+        if self.geometry_mode not in {"full-spherical", "thin-shell"}:
+            raise ValueError("geometry_mode must be 'full-spherical' or 'thin-shell'")
+        finite = (self.minimum_altitude_m, self.maximum_altitude_m, self.earth_radius_m,
+                  self.courant_factor)
+        if not all(np.isfinite(v) for v in finite):
+            raise ValueError("geometry and Courant controls must be finite")
+        if self.minimum_altitude_m >= self.maximum_altitude_m:
+            raise ValueError("altitude bounds are reversed")
+        if self.earth_radius_m + self.minimum_altitude_m <= 0.0:
+            raise ValueError("minimum radius must be positive")
+        if not 0.0 < self.courant_factor <= 1.0:
+            raise ValueError("courant_factor must be in (0, 1]")
+        if self.time_step_s is not None and (not np.isfinite(self.time_step_s)
+                                             or self.time_step_s <= 0.0):
+            raise ValueError("time_step_s must be finite and positive")
+        if self.radial_altitudes_m is not None:
+            alt = np.asarray(self.radial_altitudes_m, dtype=np.float64)
+            if (alt.ndim != 1 or len(alt) < 3 or not np.all(np.isfinite(alt))
+                    or not np.all(np.diff(alt) > 0.0)):
+                raise ValueError("radial_altitudes_m must contain at least three finite, "
+                                 "increasing values")
+            if len(alt) != self.radial_cells + 1:
+                raise ValueError("radial_cells must equal len(radial_altitudes_m) - 1")
+            if alt[0] != self.minimum_altitude_m or alt[-1] != self.maximum_altitude_m:
+                raise ValueError(
+                    "custom radial altitudes must include the configured altitude bounds")
+            steps = np.diff(alt)
+            change = self.radial_cells * np.abs(np.diff(steps)) / np.minimum(steps[:-1],
+                                                                              steps[1:])
+            if self.radial_grid_policy == "smooth" and np.any(change > 8.0 * (1.0 + 1.0e-12)):
+                raise ValueError(
+                    "custom radial grid is not smoothly graded; use a smooth node mapping or "
+                    "explicitly select radial_grid_policy='allow-abrupt'")
+            if self.radial_grid_policy == "balanced-2to1":
+                validate_radial_grid(alt, maximum_adjacent_step_ratio=2.0)
+
+
+class _SurfaceImpedanceState:
+    """Stand-in for the reference's SurfaceImpedanceADE attribute (memory view)."""
+
+    def __init__(self, owner: "GeodesicFDTD", terms: int):
+        self._owner = owner
+        self.terms = terms
+
+    @property
+    def memory(self):
+        return self._owner._si_memory
+
+    @memory.setter
+    def memory(self, values) -> None:
+        self._owner._si_memory.copy_(self._owner._tensor(values, self._owner._si_memory.dtype))
+
+    @property
+    def state_bytes(self) -> int:
+        return self._owner.backend.nbytes(self._owner._si_memory)
+
+
+class GeodesicFDTD:
+    """Earth-ionosphere FDTD on staggered geodesic radial planes (HIP backend).
+
+    ``er``/``ht`` live on integer radial planes, ``hr``/``et`` halfway between
+    them; magnetic fields lag the electric fields by half a step.
+    """
+
+    def __init__(
+        self,
+        config: SimulationConfig | None = None,
+        material: Any | None = None,
+        source: GaussianCurrent | TangentialGaussianCurrent | None = None,
+        surface_impedance: ConductiveHalfSpaceSurface | None = None,
+        plasma: MeshPlasmaModel | None = None,
+        mesh: GeodesicMesh | None = None,
+        backend: str = "hip",
+        device: str = "auto",
+        dtype: str = "auto",
+        compile_step: bool = False,
+        compile_chunk_size: int = 8,
+        torch_threads: int | None = None,
+        storage_scales: dict[str, float] | None = None,
+    ) -> None:
+        if (isinstance(compile_chunk_size, bool)
+                or not isinstance(compile_chunk_size, (int, np.integer))
+                or compile_chunk_size < 1):
+            raise ValueError("compile_chunk_size must be a positive integer")
+        self.compile_chunk_size = int(compile_chunk_size)
+        self.config = config or SimulationConfig()
+        cfg = self.config
+        if cfg.radial_material_support != "point" or cfg.tangential_material_support != "point" \
+                or cfg.horizontal_anomaly_mode != "point":
+            raise ValueError("this package supports point material sampling only")
+        if mesh is None:
+            self.mesh = build_geodesic_mesh(cfg.subdivision, cfg.mesh_relaxations,
+                                            cfg.mesh_orientation, cfg.mesh_optimization_steps)
+        else:
+            if cfg.mesh_relaxations or cfg.mesh_optimization_steps:
+                raise ValueError("mesh relaxation and optimization controls cannot accompany "
+                                 "a provided mesh")
+            if mesh.topology_kind == "uniform":
+                polar = np.count_nonzero((mesh.vertex_degree == 5) & np.isclose(
+                    np.abs(mesh.vertices[:, 2]), 1.0, rtol=0.0, atol=1.0e-13))
+                if polar != (2 if cfg.mesh_orientation == "polar" else 0):
+                    raise ValueError("provided mesh orientation does not match config")
+            self.mesh = mesh
+        if self.mesh.subdivision is not None and self.mesh.subdivision != cfg.subdivision:
+            raise ValueError("provided mesh subdivision does not match config")
+        self.backend: ArrayBackend = create_backend(backend, self.mesh, device=device,
+                                                    dtype=dtype, torch_threads=torch_threads)
+        self._lib = _native.load_library()
+        self.material = material or EarthIonosphereMaterial()
+        self.source = source
+        self.surface_impedance = surface_impedance
+        self.plasma = plasma
+        if cfg.radial_boundary_condition == "surface-impedance":
+            if surface_impedance is None:
+                raise ValueError("surface-impedance boundary requires a surface model")
+        elif surface_impedance is not None:
+            raise ValueError("surface impedance model requires radial_boundary_condition="
+                             "'surface-impedance'")
+        # radial grid (solver.py:260-283)
+        if cfg.radial_altitudes_m is None:
+            self.altitudes_m = np.linspace(cfg.minimum_altitude_m, cfg.maximum_altitude_m,
+                                           cfg.radial_cells + 1)
+        else:
+            self.altitudes_m = np.asarray(cfg.radial_altitudes_m, dtype=np.float64)
+        self.radii_m = cfg.earth_radius_m + self.altitudes_m
+        self.radial_midpoints_m = 0.5 * (self.radii_m[:-1] + self.radii_m[1:])
+        self.radial_midpoint_altitudes_m = self.radial_midpoints_m - cfg.earth_radius_m
+        self.radial_steps_m = np.diff(self.radii_m)
+        ctrl = np.empty(len(self.radii_m))
+        ctrl[0] = 0.5 * self.radial_steps_m[0]
+        ctrl[-1] = 0.5 * self.radial_steps_m[-1]
+        ctrl[1:-1] = np.diff(self.radial_midpoints_m)
+        self.radial_node_control_lengths_m = ctrl
+        if plasma is not None:
+            plasma.validate_grid(self.mesh, self.radial_midpoint_altitudes_m)
+        (self.sigma_er, self.epsilon_r_er, self.sigma_et,
+         self.epsilon_r_et) = sample_material(self.material, self.mesh, self.altitudes_m,
+                                              self.radial_midpoint_altitudes_m,
+                                              cfg.earth_radius_m)
+        self.cfl_time_step_limit_s = cfl_time_step_limit(
+            self.mesh, self.radii_m, self.radial_steps_m, self.epsilon_r_er, self.epsilon_r_et,
+            plasma)
+        self.maximum_stable_time_step_s = cfg.courant_factor * self.cfl_time_step_limit_s
+        self.time_step_s = (cfg.time_step_s if cfg.time_step_s is not None
+                            else self.maximum_stable_time_step_s)
+        if self.time_step_s > self.maximum_stable_time_step_s * (1.0 + 1.0e-12):
+            raise ValueError(f"time step {self.time_step_s:.6e} s exceeds conservative limit "
+                             f"{self.maximum_stable_time_step_s:.6e} s")
+        if self.source is not None and \
+                self.source.carrier_frequency_hz >= 0.5 / self.time_step_s:
+            raise ValueError("source carrier frequency must be below the time-step Nyquist "
+                             "limit")
+        self._prepare_material_coefficients()
+        self._source_distribution = None
+        self._tangential_source_distribution = None
+        if isinstance(self.source, TangentialGaussianCurrent):
+            self._tangential_source_distribution = self.source.edge_distribution(self)
+        elif self.source is not None:
+            self._source_distribution = self.source.staggered_distribution(self)
+        scales = dict(DEFAULT_NARROW_SCALES)
+        scales.update(storage_scales or {})
+        self.storage_scales = scales
+        self.time_s = 0.0
+        self.steps = 0
+        self.compiled = bool(compile_step)
+        self._plan = None
+        self._keepalive: list[Any] = []
+        self._create_plan()
+
+    # ------------------------------------------------------------------ setup
+    def _prepare_material_coefficients(self) -> None:
+        """FP64 exponential (or trapezoidal) loss coefficients (solver.py:706-761)."""
+        dt = self.time_step_s
+        eps_er = EPSILON_0 * self.epsilon_r_er
+        eps_et = EPSILON_0 * self.epsilon_r_et
+        if self.config.loss_integration == "trapezoidal":
+            l_er = self.sigma_er * dt / (2.0 * eps_er)
+            l_et = self.sigma_et * dt / (2.0 * eps_et)
+            ca_er, cb_er = (1.0 - l_er) / (1.0 + l_er), dt / (eps_er * (1.0 + l_er))
+            ca_et, cb_et = (1.0 - l_et) / (1.0 + l_et), dt / (eps_et * (1.0 + l_et))
+        else:
+            ca_er, cb_er = exponential_loss_coefficients(self.sigma_er, eps_er, dt)
+            ca_et, cb_et = exponential_loss_coefficients(self.sigma_et, eps_et, dt)
+        if self.config.compress_uniform_material_coefficients:
+            ca_er = self._uniform_radial_profile(ca_er, "radial electric")
+            cb_er = self._uniform_radial_profile(cb_er, "radial electric")
+            ca_et = self._uniform_radial_profile(ca_et, "tangential electric")
+            cb_et = self._uniform_radial_profile(cb_et, "tangential electric")
+        # Host FP64 copies (the reference keeps backend arrays of the same values).
+        self._ca_er, self._cb_er, self._ca_et, self._cb_et = ca_er, cb_er, ca_et, cb_et
+        self._eps_er, self._eps_et = eps_er, eps_et
+
+    @staticmethod
+    def _uniform_radial_profile(values, label):
+        profile = values[:1]
+        if not np.array_equal(values, np.broadcast_to(profile, values.shape)):
+            raise ValueError(f"cannot compress horizontally varying {label} coefficients")
+        return profile
+
+    @staticmethod
+    def _rows(*arrays):
+        """Collapse exactly horizontally-uniform coefficient arrays to one row."""
+        rows = max(a.shape[0] for a in arrays)
+        full = [np.broadcast_to(a, (rows, a.shape[1])) for a in arrays]
+        if all(np.array_equal(a, np.broadcast_to(a[:1], a.shape)) for a in full):
+            return tuple(np.ascontiguousarray(a[:1]) for a in full)
+        return tuple(np.ascontiguousarray(a) for a in full)
+
+    def _tensor(self, values, dtype):
+        torch = self.backend.torch
+        if torch.is_tensor(values):
+            return values.to(device=self.backend.torch_device, dtype=dtype)
+        return torch.as_tensor(np.asarray(values), dtype=dtype, device=self.backend.torch_device)
+
+    def _create_plan(self) -> None:
+        """Build the C-ABI descriptors, create the plan and bind the state."""
+        torch = self.backend.torch
+        mesh = self.mesh
+        R = len(self.radial_midpoints_m)
+        keep = self._keepalive = []
+
+        def f64(a):
+            arr = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+            keep.append(arr)
+            return _native.f64_ptr(arr)
+
+        def i64(a):
+            arr = np.ascontiguousarray(np.asarray(a, dtype=np.int64))
+            keep.append(arr)
+            return _native.i64_ptr(arr)
+
+        vedges, vsigns = vertex_incidence(mesh)
+        self._vertex_edges, self._vertex_edge_signs = vedges, vsigns
+        md = _native.GeoMeshDesc(
+            n_vertices=mesh.n_vertices, n_edges=mesh.n_edges, n_faces=mesh.n_faces,
+            edges=i64(mesh.edges), faces=i64(mesh.faces), face_edges=i64(mesh.face_edges),
+            face_edge_signs=i64(mesh.face_edge_signs), edge_left_faces=i64(mesh.edge_left_faces),
+            edge_right_faces=i64(mesh.edge_right_faces), vertex_edges=i64(vedges),
+            vertex_edge_signs=i64(vsigns), max_degree=vedges.shape[1],
+            primal_edge_angles=f64(mesh.primal_edge_angles),
+            dual_edge_angles=f64(mesh.dual_edge_angles),
+            face_solid_angles=f64(mesh.face_solid_angles),
+            dual_cell_solid_angles=f64(mesh.dual_cell_solid_angles))
+        rd = _native.GeoRadialDesc(radial_cells=R, radii_m=f64(self.radii_m),
+                                   full_spherical=int(self.config.geometry_mode ==
+                                                      "full-spherical"))
+        ca_er, cb_er, eps_er = self._rows(self._ca_er, self._cb_er,
+                                          np.broadcast_to(self._eps_er, self.sigma_er.shape))
+        ca_et, cb_et, eps_et = self._rows(self._ca_et, self._cb_et,
+                                          np.broadcast_to(self._eps_et, self.sigma_et.shape))
+        storage = self.backend.dtype_name
+        narrow = storage in ("float16", "bfloat16")
+        ph = _native.GeoPhysicsDesc(
+            storage=_native.STORAGE_CODES[storage], time_step_s=self.time_step_s,
+            ca_er=f64(ca_er), cb_er=f64(cb_er), er_coef_rows=ca_er.shape[0],
+            ca_et=f64(ca_et), cb_et=f64(cb_et), et_coef_rows=ca_et.shape[0],
+            eps_er=f64(eps_er), eps_et=f64(eps_et),
+            scale_er=self.storage_scales["er"] if narrow else 1.0,
+            scale_et=self.storage_scales["et"] if narrow else 1.0,
+            scale_hr=self.storage_scales["hr"] if narrow else 1.0,
+            scale_ht=self.storage_scales["ht"] if narrow else 1.0)
+        si_terms = 0
+        if self.surface_impedance is not None:
+            tables = surface_ade_tables(self.surface_impedance, mesh.n_edges, self.time_step_s)
+            si_terms = int(self.surface_impedance.terms)
+            ph.si_terms = si_terms
+            ph.si_decay = f64(tables["decay"])
+            ph.si_drive = f64(tables["drive"])
+            ph.si_history_weights = f64(tables["history_weights"])
+            scale = tables["scale"]
+            if scale.size > 1 and np.all(scale == scale[0]):
+                scale = scale[:1]
+            ph.si_scale = f64(scale)
+            ph.si_scale_rows = scale.size
+        if self._source_distribution is not None:
+            v, l, w = self._source_distribution
+            ph.n_source = len(v)
+            ph.source_vertices, ph.source_layers, ph.source_weights = i64(v), i64(l), f64(w)
+            ph.source_length_m = self.source.vertical_element_length_m
+        if self._tangential_source_distribution is not None:
+            e, l, w = self._tangential_source_distribution
+            ph.n_tsource = len(e)
+            ph.tsource_edges, ph.tsource_layers, ph.tsource_weights = i64(e), i64(l), f64(w)
+        self._plasma_info = None
+        if self.plasma is not None:
+            pt = plasma_tables(self.plasma, mesh, self.time_step_s)
+            lo, hi = pt["band"]
+            if hi > lo:
+                ph.n_species = len(self.plasma.species)
+                ph.magnetized = int(pt["magnetized"])
+                ph.band_lo, ph.band_hi = lo, hi
+                ph.plasma_reconstruction = f64(pt["reconstruction"])
+                ph.face_centers = f64(mesh.face_centers)
+                ph.left_tangents = f64(pt["left_tangents"])
+                ph.right_tangents = f64(pt["right_tangents"])
+                ph.vertex_faces = i64(pt["vertex_faces"])
+                ph.vertex_face_weights = f64(pt["vertex_face_weights"])
+                ph.vf_width = pt["vertex_faces"].shape[1]
+                ph.plasma_coef = f64(pt["coef"])
+                ph.plasma_coef_rows = pt["coef"].shape[2]
+                ph.plasma_b = f64(pt["bdir"])
+                ph.plasma_b_rows = pt["bdir"].shape[0]
+            self._plasma_info = pt
+        plan = C.c_void_p()
+        _native.check(self._lib.geo_plan_create(C.byref(md), C.byref(rd), C.byref(ph),
+                                                C.byref(plan)), self._lib)
+        self._plan = plan
+        layout = _native.GeoLayout()
+        _native.check(self._lib.geo_plan_layout(plan, C.byref(layout)), self._lib)
+        self._layout = layout
+        dev = self.backend.torch_device
+        sdt = self.backend.storage_dtype
+        cdt = self.backend.compute_dtype
+        pn, pm = layout.node_pitch, layout.mid_pitch
+        self._er_buf = torch.zeros((mesh.n_vertices, pn), dtype=sdt, device=dev)
+        self._ht_buf = torch.zeros((mesh.n_edges, pn), dtype=sdt, device=dev)
+        self._et_buf = torch.zeros((mesh.n_edges, pm), dtype=sdt, device=dev)
+        self._hr_buf = torch.zeros((mesh.n_faces, pm), dtype=sdt, device=dev)
+        self._si_memory = (torch.zeros((mesh.n_edges, si_terms), dtype=cdt, device=dev)
+                           if si_terms else None)
+        n_species = ph.n_species
+        self._plasma_buf = (torch.zeros((n_species, 3, mesh.n_faces, layout.plasma_pitch),
+                                        dtype=cdt, device=dev) if n_species else None)
+        self._surface_impedance_ade = (_SurfaceImpedanceState(self, si_terms) if si_terms
+                                       else None)
+        self._bind()
+
+    def _bind(self) -> None:
+        st = _native.GeoState(
+            er=self._er_buf.data_ptr(), er_pitch=self._er_buf.shape[1],
+            ht=self._ht_buf.data_ptr(), ht_pitch=self._ht_buf.shape[1],
+            et=self._et_buf.data_ptr(), et_pitch=self._et_buf.shape[1],
+            hr=self._hr_buf.data_ptr(), hr_pitch=self._hr_buf.shape[1],
+            si_memory=self._si_memory.data_ptr() if self._si_memory is not None else None,
+            plasma_current=self._plasma_buf.data_ptr() if self._plasma_buf is not None else None,
+            plasma_pitch=self._layout.plasma_pitch)
+        _native.check(self._lib.geo_plan_bind(self._plan, C.byref(st)), self._lib)
+
+    def __del__(self) -> None:
+        plan = getattr(self, "_plan", None)
+        lib = getattr(self, "_lib", None)
+        if plan is not None and lib is not None:
+            try:
+                self.backend.synchronize()
+            except Exception:
+                pass
+            lib.geo_plan_destroy(plan)
+            self._plan = None
+
+    # ------------------------------------------------------------ field views
+    def _field_view(self, name: str):
+        buf = getattr(self, f"_{name}_buf")
+        cols = len(self.radii_m) if name in ("er", "ht") else len(self.radial_midpoints_m)
+        view = buf[:, :cols]
+        if self.backend.dtype_name in ("float16", "bfloat16"):
+            return view.float() / self.storage_scales[name]
+        return view
+
+    def _set_field(self, name: str, values) -> None:
+        buf = getattr(self, f"_{name}_buf")
+        cols = len(self.radii_m) if name in ("er", "ht") else len(self.radial_midpoints_m)
+        data = self._tensor(values, self.backend.compute_dtype)
+        if tuple(data.shape) != (buf.shape[0], cols):
+            raise ValueError(f"{name} must have shape {(buf.shape[0], cols)}")
+        if self.backend.dtype_name in ("float16", "bfloat16"):
+            data = data * self.storage_scales[name]
+        buf[:, :cols].copy_(data)
+
+    er = property(lambda self: self._field_view("er"),
+                  lambda self, v: self._set_field("er", v))
+    et = property(lambda self: self._field_view("et"),
+                  lambda self, v: self._set_field("et", v))
+    hr = property(lambda self: self._field_view("hr"),
+                  lambda self, v: self._set_field("hr", v))
+    ht = property(lambda self: self._field_view("ht"),
+                  lambda self, v: self._set_field("ht", v))
+
+    @property
+    def plasma_current_density(self) -> list:
+        """Per-species (face, radial, 3) current density (copies; zero off-band)."""
+        if self._plasma_buf is None:
+            if self.plasma is None:
+                return []
+            torch = self.backend.torch
+            return [torch.zeros((self.mesh.n_faces, len(self.radial_midpoints_m), 3),
+                                dtype=self.backend.compute_dtype, device=self.backend.torch_device)
+                    for _ in self.plasma.species]
+        torch = self.backend.torch
+        lo, hi = self._layout.band_lo, self._layout.band_hi
+        out = []
+        for s in range(self._plasma_buf.shape[0]):
+            full = torch.zeros((self.mesh.n_faces, len(self.radial_midpoints_m), 3),
+                               dtype=self._plasma_buf.dtype, device=self._plasma_buf.device)
+            full[:, lo:hi, :] = self._plasma_buf[s, :, :, : hi - lo].permute(1, 2, 0)
+            out.append(full)
+        return out
+
+    def set_plasma_current_density(self, index: int, values) -> None:
+        data = self._tensor(values, self.backend.compute_dtype)
+        R = len(self.radial_midpoints_m)
+        if tuple(data.shape) != (self.mesh.n_faces, R, 3):
+            raise ValueError("plasma current must have shape (face, radial, 3)")
+        if self._plasma_buf is None:
+            if bool((data != 0).any()):
+                raise ValueError("this plasma model has no drive; its current must be zero")
+            return
+        lo, hi = self._layout.band_lo, self._layout.band_hi
+        outside = torch_any_nonzero(data[:, :lo]) or torch_any_nonzero(data[:, hi:])
+        if outside:
+            raise ValueError("plasma current is nonzero outside the layers with plasma drive")
+        self._plasma_buf[index, :, :, : hi - lo] = data[:, lo:hi, :].permute(2, 0, 1)
+
+    # --------------------------------------------------------------- stepping
+    def _stream(self):
+        return C.c_void_p(self.backend.current_stream_handle())
+
+    def _source_currents(self, count: int):
+        """FP64 currents at (n + 1/2) dt for the next ``count`` steps (solver.py:851-865)."""
+        if self.source is None or count == 0:
+            return None
+        times = (self.steps + np.arange(count, dtype=np.float64) + 0.5) * self.time_step_s
+        values = self.source.currents_a(times, self.time_step_s)
+        return self._tensor(values, self.backend.compute_dtype)
+
+    def step(self, count: int = 1) -> None:
+        """Advance the fields by ``count`` complete leapfrog time steps."""
+        count = self._validated_count(count, "step count", minimum=0)
+        if count == 0:
+            return
+        currents = self._source_currents(count)
+        chunk = self.compile_chunk_size if self.compiled else 1
+        _native.check(self._lib.geo_step(
+            self._plan, None if currents is None else currents.data_ptr(), count, chunk, None,
+            self._stream()), self._lib)
+        self._pending = currents  # keep alive until the queued work has read it
+        self.steps += count
+        self.time_s = self.steps * self.time_step_s
+
+    def _advance_fields(self, current_a: float = 0.0) -> None:
+        """One leapfrog step with an explicit source current (reference signature)."""
+        value = float(current_a)
+        currents = self._tensor(np.asarray([value]), self.backend.compute_dtype)
+        _native.check(self._lib.geo_step(self._plan, currents.data_ptr(), 1, 1, None,
+                                         self._stream()), self._lib)
+        self._pending = currents
+
+    def record_er_observations(self, vertex_indices, radial_layers, weights, steps: int, *,
+                               synchronize_every: int = 128) -> NDArray[np.generic]:
+        """Advance and record weighted Er samples (solver.py:1080-1136).
+
+        Row 0 is the initial field and row n the field after step n.  The
+        samples are written by the step kernels into a device trace buffer;
+        one device-to-host copy happens at the end.
+        """
+        vertices = self._validated_index_array(vertex_indices, "vertex_indices")
+        layers = self._validated_index_array(radial_layers, "radial_layers")
+        w = np.asarray(weights, dtype=np.float64)
+        steps = self._validated_count(steps, "step count", minimum=0)
+        self._validated_count(synchronize_every, "synchronize_every", minimum=1)
+        if vertices.ndim != 2 or w.shape != vertices.shape:
+            raise ValueError("vertex_indices and weights must have matching 2-D shapes")
+        if layers.shape != (vertices.shape[0],):
+            raise ValueError("radial_layers must contain one layer per observation")
+        if np.any(vertices < 0) or np.any(vertices >= self.mesh.n_vertices):
+            raise ValueError("observation vertex index is out of range")
+        if np.any(layers < 0) or np.any(layers >= len(self.radii_m)):
+            raise ValueError("observation radial layer is out of range")
+        if not np.all(np.isfinite(w)):
+            raise ValueError("observation weights must be finite")
+        if not np.allclose(w.sum(axis=1), 1.0):
+            raise ValueError("observation weights must sum to one")
+        n_rx = vertices.shape[0]
+        width = vertices.shape[1]
+        if width > 3:
+            raise ValueError("each observation may combine at most three vertices")
+        v3 = np.zeros((n_rx, 3), dtype=np.int64)
+        w3 = np.zeros((n_rx, 3), dtype=np.float64)
+        v3[:, :width] = vertices
+        v3[:, width:] = vertices[:, :1]
+        w3[:, :width] = w
+        torch = self.backend.torch
+        traces = torch.zeros((steps + 1, n_rx), dtype=self.backend.compute_dtype,
+                             device=self.backend.torch_device)
+        if n_rx == 0:
+            self.step(steps)
+            return self.to_numpy(traces)
+        v3c, l3c, w3c = (np.ascontiguousarray(v3), np.ascontiguousarray(layers),
+                         np.ascontiguousarray(w3))
+        _native.check(self._lib.geo_set_receivers(self._plan, _native.i64_ptr(v3c),
+                                                  _native.i64_ptr(l3c), _native.f64_ptr(w3c),
+                                                  n_rx), self._lib)
+        currents = self._source_currents(steps)
+        chunk = self.compile_chunk_size if self.compiled else 1
+        _native.check(self._lib.geo_step(
+            self._plan, None if currents is None else currents.data_ptr(), steps, chunk,
+            traces.data_ptr(), self._stream()), self._lib)
+        self.steps += steps
+        self.time_s = self.steps * self.time_step_s
+        self.backend.synchronize()
+        return self.to_numpy(traces)
+
+    def record_h_observations(self, face_indices, face_radial_layers, face_weights,
+                              edge_indices, edge_radial_layers, edge_weights, steps: int, *,
+                              synchronize_every: int = 128, sample_every: int = 1):
+        """Weighted Hr/Ht samples at half-step times (solver.py:1138-1229)."""
+        faces = self._validated_index_array(face_indices, "face_indices")
+        flayers = self._validated_index_array(face_radial_layers, "face_radial_layers")
+        fw = np.asarray(face_weights, dtype=np.float64)
+        edges = self._validated_index_array(edge_indices, "edge_indices")
+        elayers = self._validated_index_array(edge_radial_layers, "edge_radial_layers")
+        ew = np.asarray(edge_weights, dtype=np.float64)
+        steps = self._validated_count(steps, "step count", minimum=0)
+        self._validated_count(synchronize_every, "synchronize_every", minimum=1)
+        sample_every = self._validated_count(sample_every, "sample_every", minimum=1)
+        if faces.ndim != 2 or fw.shape != faces.shape:
+            raise ValueError("face indices and weights must have matching 2-D shapes")
+        if edges.ndim != 2 or ew.shape != edges.shape:
+            raise ValueError("edge indices and weights must have matching 2-D shapes")
+        if flayers.shape != faces.shape:
+            raise ValueError("face radial layers must match the face indices")
+        if elayers.shape != edges.shape:
+            raise ValueError("edge radial layers must match the edge indices")
+        if np.any(faces < 0) or np.any(faces >= self.mesh.n_faces):
+            raise ValueError("observation face index is out of range")
+        if np.any(edges < 0) or np.any(edges >= self.mesh.n_edges):
+            raise ValueError("observation edge index is out of range")
+        if np.any(flayers < 0) or np.any(flayers >= len(self.radial_midpoints_m)):
+            raise ValueError("radial H observation layer is out of range")
+        if np.any(elayers < 0) or np.any(elayers >= len(self.radii_m)):
+            raise ValueError("tangential H observation layer is out of range")
+        if not np.all(np.isfinite(fw)) or not np.all(np.isfinite(ew)):
+            raise ValueError("observation weights must be finite")
+        torch = self.backend.torch
+        dev = self.backend.torch_device
+        cdt = self.backend.compute_dtype
+        tf, tfl, tfw = (torch.as_tensor(faces, device=dev), torch.as_tensor(flayers, device=dev),
+                        torch.as_tensor(fw, dtype=cdt, device=dev))
+        te, tel, tew = (torch.as_tensor(edges, device=dev), torch.as_tensor(elayers, device=dev),
+                        torch.as_tensor(ew, dtype=cdt, device=dev))
+        sample_steps = np.unique(np.concatenate((np.arange(0, steps + 1, sample_every),
+                                                 np.asarray((steps,)))))
+        rt = torch.zeros((len(sample_steps), faces.shape[0]), dtype=cdt, device=dev)
+        tt = torch.zeros((len(sample_steps), edges.shape[0]), dtype=cdt, device=dev)
+
+        def sample(row: int) -> None:
+            hr = self.hr
+            ht = self.ht
+            rt[row] = (hr[tf, tfl].to(cdt) * tfw).sum(dim=1)
+            tt[row] = (ht[te, tel].to(cdt) * tew).sum(dim=1)
+
+        start = self.steps
+        sample(0)
+        for row, target in enumerate(sample_steps[1:], start=1):
+            self.step(int(target) + start - self.steps)
+            sample(row)
+        self.backend.synchronize()
+        return self.to_numpy(rt), self.to_numpy(tt)
+
+    # ---------------------------------------------------------- diagnostics
+    def field_diagnostics(self) -> dict[str, float]:
+        """FP64 device reductions: max |field|, staggered energies, non-finite count."""
+        out = _native.GeoDiagOut()
+        _native.check(self._lib.geo_diagnostics(self._plan, C.byref(out), self._stream()),
+                      self._lib)
+        return {name: getattr(out, name) for name, _ in out._fields_}
+
+    def energy_j(self) -> float:
+        """Total staggered field energy (physics_diagnostics/model.py:242-258 weights)."""
+        d = self.field_diagnostics()
+        return d["energy_er"] + d["energy_et"] + d["energy_hr"] + d["energy_ht"]
+
+    def diagnostics(self) -> dict[str, float | int | str]:
+        """Scalar diagnostics without saving field data (solver.py:991-1031)."""
+        d = self.field_diagnostics()
+        return {
+            "step": self.steps,
+            "time_s": self.time_s,
+            "electric_time_s": self.electric_time_s,
+            "magnetic_time_s": self.magnetic_time_s,
+            "backend": self.backend.name,
+            "device": self.backend.device,
+            "dtype": self.backend.dtype_name,
+            "compiled": self.compiled,
+            "compile_chunk_size": self.compile_chunk_size,
+            "radial_boundary_condition": self.config.radial_boundary_condition,
+            "surface_impedance_terms": (self.surface_impedance.terms
+                                        if self.surface_impedance is not None else 0),
+            "surface_impedance_state_bytes": (self.backend.nbytes(self._si_memory)
+                                              if self._si_memory is not None else 0),
+            "plasma_species": len(self.plasma.species) if self.plasma else 0,
+            "plasma_state_bytes": (self.backend.nbytes(self._plasma_buf)
+                                   if self._plasma_buf is not None else 0),
+            "loss_integration": self.config.loss_integration,
+            "geometry_mode": self.config.geometry_mode,
+            "cfl_time_step_limit_s": self.cfl_time_step_limit_s,
+            "courant_factor": self.config.courant_factor,
+            "field_memory_bytes": self.memory_bytes,
+            "persistent_backend_bytes": self.persistent_backend_bytes,
+            "max_abs_er_v_m": d["max_abs_er"],
+            "max_abs_et_v_m": d["max_abs_et"],
+            "max_abs_hr_a_m": d["max_abs_hr"],
+            "max_abs_ht_a_m": d["max_abs_ht"],
+            "energy_j": d["energy_er"] + d["energy_et"] + d["energy_hr"] + d["energy_ht"],
+            "field_finite": d["nonfinite"] == 0,
+        }
+
+    @property
+    def electric_time_s(self) -> float:
+        return self.steps * self.time_step_s
+
+    @property
+    def magnetic_time_s(self) -> float:
+        return (self.steps - 0.5) * self.time_step_s
+
+    def to_numpy(self, values: Any) -> NDArray[np.generic]:
+        return self.backend.to_numpy(values)
+
+    def field_value(self, field: str, *indices: int) -> float:
+        if field not in {"er", "et", "hr", "ht"}:
+            raise ValueError("field must be er, et, hr, or ht")
+        return self.backend.scalar(getattr(self, field)[indices])
+
+    @property
+    def memory_bytes(self) -> int:
+        """Bytes of the four padded field buffers."""
+        return sum(self.backend.nbytes(b) for b in (self._er_buf, self._et_buf, self._hr_buf,
+                                                    self._ht_buf))
+
+    @property
+    def persistent_backend_bytes(self) -> int:
+        total = self.memory_bytes
+        for b in (self._si_memory, self._plasma_buf):
+            if b is not None:
+                total += self.backend.nbytes(b)
+        return total
+
+    def kernel_times_ms(self, steps: int) -> tuple[float, float]:
+        """Average (H, E) kernel durations of ``steps`` event-timed single steps.
+
+        The timed steps advance the state with a zero source current."""
+        out = (C.c_double * 2)()
+        _native.check(self._lib.geo_time_kernels(self._plan, steps, self._stream(), out),
+                      self._lib)
+        self.steps += steps
+        self.time_s = self.steps * self.time_step_s
+        return float(out[0]), float(out[1])
+
+    def save_checkpoint(self, path: str | Path) -> Path:
+        from .checkpoint import save_checkpoint
+
+        return save_checkpoint(self, path)
+
+    @classmethod
+    def load_checkpoint(cls, path: str | Path, *, backend: str = "hip", device: str = "auto",
+                        dtype: str | None = None, compile_step: bool = False,
+                        compile_chunk_size: int = 8, torch_threads: int | None = None):
+        from .checkpoint import load_checkpoint
+
+        return load_checkpoint(path, backend=backend, device=device, dtype=dtype,
+                               compile_step=compile_step, compile_chunk_size=compile_chunk_size,
+                               torch_threads=torch_threads)
+
+    @staticmethod
+    def _validated_count(value: int, label: str, *, minimum: int) -> int:
+        if isinstance(value, bool) or not isinstance(value, (int, np.integer)):
+            raise ValueError(f"{label} must be an integer")
+        result = int(value)
+        if result < minimum:
+            raise ValueError(f"{label} must be {'non-negative' if minimum == 0 else 'positive'}")
+        return result
+
+    @staticmethod
+    def _validated_index_array(values: Any, label: str) -> NDArray[np.int64]:
+        array = np.asarray(values)
+        if np.issubdtype(array.dtype, np.bool_) or not np.issubdtype(array.dtype, np.integer):
+            raise ValueError(f"{label} must contain integers")
+        return np.asarray(array, dtype=np.int64)
+
+
+def _validated_material(values, shape, label):
+    try:
+        sigma, eps = values
+    except (TypeError, ValueError) as error:
+        raise ValueError(f"{label} material sampler must return conductivity and "
+                         "permittivity") from error
+    sigma = np.asarray(sigma, dtype=np.float64)
+    eps = np.asarray(eps, dtype=np.float64)
+    if sigma.shape != shape or eps.shape != shape:
+        raise ValueError(f"{label} material arrays must have shape {shape}")
+    if not np.all(np.isfinite(sigma)) or not np.all(np.isfinite(eps)):
+        raise ValueError(f"{label} material arrays must be finite")
+    if np.any(sigma < 0.0):
+        raise ValueError(f"{label} conductivity cannot be negative")
+    if np.any(eps <= 0.0):
+        raise ValueError(f"{label} relative permittivity must be positive")
+    return sigma, eps
+
+
+def sample_material(material, mesh, altitudes_m, midpoint_altitudes_m, earth_radius_m):
+    """Point-sampled (sigma_er, eps_r_er, sigma_et, eps_r_et) (solver.py:467-629).
+
+    Er samples sit on vertices x radial nodes, Et samples on edge midpoints x
+    radial midpoints; a material with a ``sample_mesh`` hook provides all four.
+    """
+    shape_er = (mesh.n_vertices, len(altitudes_m))
+    shape_et = (mesh.n_edges, len(midpoint_altitudes_m))
+    sampler = getattr(material, "sample_mesh", None)
+    if sampler is not None:
+        values = sampler(mesh, altitudes_m, earth_radius_m, radial_material_support="point",
+                         tangential_material_support="point", horizontal_anomaly_mode="point")
+        try:
+            s_er, e_er, s_et, e_et = values
+        except (TypeError, ValueError) as error:
+            raise ValueError("mesh material sampler must return four property arrays") \
+                from error
+        return (*_validated_material((s_er, e_er), shape_er, "mesh-native radial"),
+                *_validated_material((s_et, e_et), shape_et, "mesh-native tangential"))
+    return (*_validated_material(material.sample(mesh.vertices, altitudes_m, earth_radius_m),
+                                 shape_er, "radial"),
+            *_validated_material(material.sample(mesh.edge_midpoints(), midpoint_altitudes_m,
+                                                 earth_radius_m), shape_et, "tangential"))
+
+
+def cfl_time_step_limit(mesh, radii_m, radial_steps_m, epsilon_r_er, epsilon_r_et,
+                        plasma=None) -> float:
+    """Conservative lossless CFL limit, plasma-limited (solver.py:377-406)."""
+    smallest = float(np.min(radii_m))
+    primal = smallest * float(mesh.primal_edge_angles.min())
+    dual = smallest * float(mesh.dual_edge_angles.min())
+    radial = float(np.min(radial_steps_m))
+    inv_len2 = primal**-2 + dual**-2 + (2.0 / radial) ** 2
+    min_eps = min(float(np.min(epsilon_r_er)), float(np.min(epsilon_r_et)))
+    geometric = np.sqrt(min_eps) / (C_0 * np.sqrt(inv_len2))
+    if plasma is None:
+        return geometric
+    wp = np.sqrt(plasma.maximum_plasma_drive() / (EPSILON_0 * min_eps))
+    if wp == 0.0:
+        return geometric
+    return min(geometric, 2.0 / wp)
+
+
+def stable_time_step(config: SimulationConfig, mesh, material=None, plasma=None) -> float:
+    """The time step GeodesicFDTD would choose, without creating any device state."""
+    material = material or EarthIonosphereMaterial()
+    if config.radial_altitudes_m is None:
+        altitudes = np.linspace(config.minimum_altitude_m, config.maximum_altitude_m,
+                                config.radial_cells + 1)
+    else:
+        altitudes = np.asarray(config.radial_altitudes_m, dtype=np.float64)
+    radii = config.earth_radius_m + altitudes
+    mids = 0.5 * (radii[:-1] + radii[1:])
+    _, e_er, _, e_et = sample_material(material, mesh, altitudes, mids - config.earth_radius_m,
+                                       config.earth_radius_m)
+    limit = cfl_time_step_limit(mesh, radii, np.diff(radii), e_er, e_et, plasma)
+    return config.time_step_s if config.time_step_s is not None else config.courant_factor * limit
+
+
+def torch_any_nonzero(t) -> bool:
+    return bool(t.numel()) and bool((t != 0).any())
+
+
+def exponential_loss_coefficients(sigma, epsilon, dt):
+    """Exact conductive decay with midpoint forcing: Ca = exp(-q),
+    Cb = dt/eps * (-expm1(-q)/q) (= dt/eps at q = 0), q = sigma dt / eps."""
+    rate = sigma * dt / epsilon
+    decay = np.exp(-rate)
+    phi1 = np.ones_like(rate)
+    nz = rate != 0.0
+    phi1[nz] = -np.expm1(-rate[nz]) / rate[nz]
+    return decay, dt / epsilon * phi1

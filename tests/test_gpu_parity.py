@@ -1,0 +1,167 @@
+"""HIP step kernels vs the CPU oracle on the same seeded inputs.
+
+Bars (stated per test): FP64 storage and arithmetic within 1e-10 relative L2
+of the FP64 oracle (roundoff only: the kernels fold coefficients in a
+different order); FP32 within 1e-5 relative L2 of the FP64 oracle (the
+north-star FP32 tolerance; the reference's own FP32-vs-FP64 test allows
+2e-5, tests/test_backends.py:262-293); integer work (mesh tables, sampling
+indices) bit-exact; repeated runs bitwise identical.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import scenarios as S
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("er", "et", "hr", "ht")
+FP64_BAR = 1e-10
+FP32_BAR = 1e-5
+
+
+def _seed_fields(sim, ref, seed=20260814):
+    shapes = {name: tuple(getattr(ref, name).shape) for name in FIELDS}
+    values = S.seeded_fields(shapes, seed=seed)
+    for name in FIELDS:
+        getattr(ref, name)[:] = values[name]
+        setattr(sim, name, values[name])
+
+
+def _compare(sim, ref, bar, label):
+    worst = {}
+    for name in FIELDS:
+        got = sim.to_numpy(getattr(sim, name))
+        want = getattr(ref, name)
+        assert got.shape == want.shape
+        worst[name] = S.rel_l2(got, want)
+    print(label, {k: f"{v:.2e}" for k, v in worst.items()}, flush=True)
+    for name, err in worst.items():
+        assert err <= bar, f"{label}: {name} rel-L2 {err:.3e} > {bar:g}"
+    return worst
+
+
+@pytest.mark.parametrize("name,steps", [
+    ("c0_pulse", 200), ("c0_plasma", 200), ("magnetized", 150), ("thin_trapezoid", 150),
+    ("tangential", 150),
+])
+def test_fp64_fields_and_traces_match_oracle(name, steps):
+    sc = S.scenario(name)
+    sim = S.build_sim(sc)
+    ref = S.build_oracle(sc)
+    assert sim.time_step_s == ref.time_step_s
+    pts = S.receiver_points(sc)
+    v, l, w = S.receiver_tables(sim.mesh, sim.altitudes_m, pts)
+    traces = sim.record_er_observations(v, l, w, steps)
+    want = ref.record_er_observations(v, l, w, steps)
+    _compare(sim, ref, FP64_BAR, f"{name} fp64")
+    err = S.rel_l2(traces, want)
+    print(name, "trace rel-L2", f"{err:.2e}", "peak", float(np.max(np.abs(want))), flush=True)
+    assert err <= FP64_BAR
+    if sim.plasma is not None:
+        got_j = sim.to_numpy(sim.plasma_current_density[0])
+        want_j = ref.coupler.current_density[0]
+        assert S.rel_l2(got_j, want_j) <= FP64_BAR
+
+
+def test_fp64_random_fields_pec_vacuum():
+    """The reference benchmark workload: vacuum, PEC, random fields x 1e-6."""
+    sc = S.scenario("pec_vacuum")
+    sim = S.build_sim(sc)
+    ref = S.build_oracle(sc)
+    _seed_fields(sim, ref)
+    sim.step(100)
+    ref.step(100)
+    _compare(sim, ref, FP64_BAR, "pec_vacuum fp64")
+
+
+@pytest.mark.parametrize("name,steps", [("c0_pulse", 200), ("c0_plasma", 200),
+                                        ("pec_vacuum", 100)])
+def test_fp32_within_tolerance_of_fp64_oracle(name, steps):
+    sc = S.scenario(name)
+    sim = S.build_sim(sc, dtype="float32")
+    ref = S.build_oracle(sc)
+    if name == "pec_vacuum":
+        _seed_fields(sim, ref)
+    sim.step(steps)
+    ref.step(steps)
+    _compare(sim, ref, FP32_BAR, f"{name} fp32")
+
+
+def test_c0_full_run_1000_steps_fp64():
+    """C0 as specified: s2 x 24, SI ground, 20 Hz source, 4 receivers, 1000 steps."""
+    sc = S.scenario("c0")
+    sim = S.build_sim(sc, compile_step=True, compile_chunk_size=64)
+    ref = S.build_oracle(sc)
+    pts = S.receiver_points(sc)
+    v, l, w = S.receiver_tables(sim.mesh, sim.altitudes_m, pts)
+    traces = sim.record_er_observations(v, l, w, 1000)
+    want = ref.record_er_observations(v, l, w, 1000)
+    _compare(sim, ref, FP64_BAR, "c0 1000 fp64")
+    assert S.rel_l2(traces, want) <= FP64_BAR
+
+
+def test_graph_chunks_equal_single_launches_bitwise():
+    sc = S.scenario("c0_plasma")
+    a = S.build_sim(sc, dtype="float32")
+    b = S.build_sim(sc, dtype="float32", compile_step=True, compile_chunk_size=16)
+    a.step(37)
+    b.step(37)  # two graph replays + 5 single steps
+    for name in FIELDS:
+        assert np.array_equal(a.to_numpy(getattr(a, name)), b.to_numpy(getattr(b, name))), name
+
+
+def test_repeated_runs_are_bitwise_identical():
+    sc = S.scenario("c0_plasma")
+    outs = []
+    for _ in range(2):
+        sim = S.build_sim(sc, dtype="float32", compile_step=True)
+        sim.step(50)
+        outs.append([sim.to_numpy(getattr(sim, n)).copy() for n in FIELDS])
+    for x, y in zip(*outs):
+        assert np.array_equal(x, y)
+
+
+def test_padding_stays_zero_and_views_alias_device_state():
+    sc = S.scenario("c0_pulse")
+    sim = S.build_sim(sc, dtype="float32")
+    sim.step(20)
+    for name in FIELDS:
+        buf = getattr(sim, f"_{name}_buf")
+        cols = getattr(sim, name).shape[1]
+        assert bool((buf[:, cols:] == 0).all()), name
+    sim.er[:, 3] = 1.0
+    assert float(sim._er_buf[:, 3].min()) == 1.0
+
+
+def test_step_count_chunk_remainder_semantics():
+    """step(n) = full graph chunks + single-step remainder (solver.py:793-804)."""
+    sc = S.scenario("c0_pulse")
+    a = S.build_sim(sc)
+    b = S.build_sim(sc, compile_step=True, compile_chunk_size=8)
+    ref = S.build_oracle(sc)
+    for n in (19, 8, 3):
+        a.step(n)
+        b.step(n)
+        ref.step(n)
+    assert a.steps == b.steps == ref.steps == 30
+    assert a.time_s == pytest.approx(30 * a.time_step_s, rel=1e-15)
+    _compare(b, ref, FP64_BAR, "chunked 19+8+3")
+
+
+def test_energy_diagnostic_matches_oracle_fp64_reduction():
+    sc = S.scenario("pec_vacuum")
+    sim = S.build_sim(sc)
+    ref = S.build_oracle(sc)
+    _seed_fields(sim, ref)
+    sim.step(10)
+    ref.step(10)
+    o = S.oracle_module()
+    got = sim.energy_j()
+    want = o.energy(ref)
+    assert got == pytest.approx(want, rel=1e-12)
+    d = sim.diagnostics()
+    assert d["max_abs_er_v_m"] == pytest.approx(float(np.max(np.abs(ref.er))), rel=1e-14)
+    assert d["field_finite"]
