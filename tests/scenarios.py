@@ -63,6 +63,7 @@ def scenario(name: str, **overrides) -> dict:
         "source": {"latitude_deg": SOURCE_LAT, "longitude_deg": SOURCE_LON, "altitude_m": 0.0,
                    "carrier_frequency_hz": 20.0},
         "tangential": None, "dtype": "float64", "receivers": "c0", "time_step_s": None,
+        "seed_fields": False,
     }
     if name == "c0":
         pass
@@ -74,6 +75,13 @@ def scenario(name: str, **overrides) -> dict:
         base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3}
         base["source"] = dict(base["source"], one_over_e_half_width_s="30dt",
                               center_time_s="90dt")
+    elif name == "plasma_random":
+        # seeded fields everywhere so the plasma band is driven from the first step
+        base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3}
+        base["seed_fields"] = True
+    elif name == "magnetized_random":
+        base.update(subdivision=1, radial_cells=12, seed_fields=True)
+        base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3, "magnetized": True}
     elif name == "magnetized":
         base.update(subdivision=1, radial_cells=12)
         base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3, "magnetized": True}
@@ -81,14 +89,16 @@ def scenario(name: str, **overrides) -> dict:
                               center_time_s="60dt")
     elif name == "pec_vacuum":
         base.update(vacuum=True, surface_sigma=None, source=None, minimum_altitude_m=0.0,
-                    subdivision=2, radial_cells=16)
+                    subdivision=2, radial_cells=16, seed_fields=True)
     elif name == "thin_trapezoid":
         base.update(geometry_mode="thin-shell", loss_integration="trapezoidal",
                     surface_sigma=None, subdivision=1, radial_cells=10)
         base["source"] = dict(base["source"], altitude_m=2_500.0,
                               one_over_e_half_width_s="20dt", center_time_s="60dt")
     elif name == "tangential":
-        base.update(subdivision=2, radial_cells=12, surface_sigma=None)
+        # midpoints must straddle the 0 m line source (reference sources.py:138-139)
+        base.update(subdivision=2, radial_cells=11, surface_sigma=None,
+                    minimum_altitude_m=-10e3)
         base["source"] = None
         base["tangential"] = {"latitude_deg": 46.5, "longitude_deg": -90.9, "altitude_m": 0.0,
                               "azimuths_deg": (0.0, 90.0), "line_lengths_m": (22_500.0, 22_500.0),
@@ -277,6 +287,16 @@ def attach_tangential_source(sim, params):
     return sim
 
 
+def apply_seed_fields(sim, set_field):
+    """Seed the four fields of a scenario with seed_fields=True (same values on
+    every implementation): set_field(name, values)."""
+    R = sim.radial_midpoints_m.shape[0]
+    shapes = {"er": (sim.mesh.n_vertices, R + 1), "et": (sim.mesh.n_edges, R),
+              "hr": (sim.mesh.n_faces, R), "ht": (sim.mesh.n_edges, R + 1)}
+    for name, values in seeded_fields(shapes).items():
+        set_field(name, values)
+
+
 def seeded_fields(shape_dict, seed=20260814, scale=1e-6):
     """Random initial fields like the reference benchmark (backend_matrix.py:179-183)."""
     rng = np.random.default_rng(seed)
@@ -288,6 +308,27 @@ def rel_l2(a, b) -> float:
     b = np.asarray(b, dtype=np.float64)
     den = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / den) if den > 0 else float(np.linalg.norm(a - b))
+
+
+def family_errors(got: dict, want: dict) -> dict:
+    """Per-field error relative to the norm of its field family.
+
+    A field that is only roundoff in an excitation (Hr under a vertical
+    source: Et stays a discrete gradient, so its face circulation vanishes)
+    has no meaningful relative error of its own; measuring it against the
+    (Er, Et) or (Hr, Ht) family norm keeps the bar meaningful.
+    """
+    norms = {}
+    for fam in (("er", "et"), ("hr", "ht")):
+        n = np.sqrt(sum(float(np.sum(np.asarray(want[k], dtype=np.float64) ** 2)) for k in fam))
+        for k in fam:
+            norms[k] = n
+    out = {}
+    for k in ("er", "et", "hr", "ht"):
+        diff = np.linalg.norm(np.asarray(got[k], dtype=np.float64) - np.asarray(want[k],
+                                                                                dtype=np.float64))
+        out[k] = float(diff / norms[k]) if norms[k] > 0 else float(diff)
+    return out
 
 
 def gpu_available() -> bool:

@@ -1,0 +1,229 @@
+"""Host-side setup of the product package and the C-ABI boundary (CPU only).
+
+The product builds its tables with vectorised code of its own; these tests
+pin them against the reference fixtures (bit-exact mesh arrays at every
+level used by the benchmark configurations, source / receiver
+distributions, time steps) and against the oracle (surface-impedance and
+plasma tables), and check that the shared library loads and exports every
+function declared in include/geofdtd.h without touching a GPU.
+"""
+
+from __future__ import annotations
+
+import hashlib
+import json
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import scenarios as S
+
+import ionosphere_fdtd_amd as g
+from ionosphere_fdtd_amd import _native
+from ionosphere_fdtd_amd.plasma import plasma_tables
+from ionosphere_fdtd_amd.solver import stable_time_step
+from ionosphere_fdtd_amd.surface_impedance import surface_ade_tables
+
+GOLDEN = Path(__file__).resolve().parent / "golden"
+REPO = Path(__file__).resolve().parents[1]
+o = S.oracle_module()
+INT_FIELDS = ("faces", "edges", "face_edges", "face_edge_signs", "edge_left_faces",
+              "edge_right_faces", "vertex_degree")
+FLOAT_FIELDS = ("vertices", "face_centers", "primal_edge_angles", "dual_edge_angles",
+                "face_solid_angles", "dual_cell_solid_angles")
+
+
+def _sha(values):
+    a = np.asarray(values)
+    dt = "<f8" if np.issubdtype(a.dtype, np.floating) else "<i8"
+    return hashlib.sha256(np.asarray(a, dtype=dt, order="C").tobytes()).hexdigest()
+
+
+@pytest.mark.parametrize("s", range(8))
+def test_product_mesh_is_bit_exact_with_reference(s):
+    """Integer ordering and FP64 geometry identical to the reference, s0..s7."""
+    hashes = json.loads((GOLDEN / "mesh_hashes.json").read_text())[f"s{s}_polar"]
+    mesh = g.build_geodesic_mesh(s)
+    for f in INT_FIELDS + FLOAT_FIELDS:
+        assert _sha(getattr(mesh, f)) == hashes[f], f
+
+
+@pytest.mark.parametrize("s", range(4))
+def test_product_native_orientation_matches_reference(s):
+    fx = np.load(GOLDEN / "mesh_s0_s3.npz")
+    mesh = g.build_geodesic_mesh(s, orientation="native")
+    for f in INT_FIELDS:
+        assert np.array_equal(getattr(mesh, f), fx[f"s{s}_native_{f}"]), f
+
+
+def test_vertex_incidence_matches_oracle():
+    mesh = g.build_geodesic_mesh(3)
+    from ionosphere_fdtd_amd.mesh import vertex_incidence
+
+    ve, vs = vertex_incidence(mesh)
+    oe, os_ = o.vertex_incidence(o.build_mesh(3))
+    assert np.array_equal(ve, oe)
+    assert np.array_equal(vs, os_.astype(np.int64))
+
+
+@pytest.mark.parametrize("name", ("c0", "c0_plasma", "magnetized", "tangential",
+                                  "thin_trapezoid", "pec_vacuum"))
+def test_time_step_and_distributions_match_reference(name):
+    sc = S.scenario(name)
+    fx = np.load(GOLDEN / f"trajectory_{name}.npz")
+    mesh = g.build_geodesic_mesh(sc["subdivision"])
+    cfg = g.SimulationConfig(subdivision=sc["subdivision"], radial_cells=sc["radial_cells"],
+                             minimum_altitude_m=sc["minimum_altitude_m"],
+                             maximum_altitude_m=sc["maximum_altitude_m"],
+                             earth_radius_m=sc["earth_radius_m"],
+                             courant_factor=sc["courant_factor"])
+    material = S.VacuumMaterial() if sc["vacuum"] else g.EarthIonosphereMaterial(**sc["material"])
+    plasma = None
+    if sc["plasma"]:
+        alt = S._midpoint_altitudes(sc)
+        b, species = S._plasma_arrays(sc, mesh, alt)
+        plasma = g.MeshPlasmaModel.from_mesh(mesh, alt, b,
+                                             tuple(g.ColdPlasmaSpecies(*s) for s in species))
+    assert stable_time_step(cfg, mesh, material, plasma) == float(fx["time_step_s"])
+    altitudes = np.linspace(sc["minimum_altitude_m"], sc["maximum_altitude_m"],
+                            sc["radial_cells"] + 1)
+    verts, layers, weights = S.receiver_tables(mesh, altitudes,
+                                               [tuple(p) for p in fx["receiver_points"]])
+    assert np.array_equal(verts, fx["receiver_vertices"])
+    assert np.array_equal(layers, fx["receiver_layers"])
+    np.testing.assert_allclose(weights, fx["receiver_weights"], rtol=0, atol=1e-15)
+    if "source_vertices" in fx:
+        from ionosphere_fdtd_amd.sources import barycentric_distribution, radial_linear_distribution
+
+        src = sc["source"]
+        v, _, hw = barycentric_distribution(mesh, altitudes, src["latitude_deg"],
+                                            src["longitude_deg"], src["altitude_m"])
+        lay, rw = radial_linear_distribution(altitudes, src["altitude_m"])
+        n = len(lay)
+        assert np.array_equal(np.repeat(v, n), fx["source_vertices"])
+        assert np.array_equal(np.tile(lay, len(v)), fx["source_layers"])
+        np.testing.assert_allclose(np.repeat(hw, n) * np.tile(rw, len(v)), fx["source_weights"],
+                                   rtol=0, atol=1e-15)
+
+
+def test_source_waveform_vectorised_equals_scalar_form():
+    src = g.GaussianCurrent(carrier_frequency_hz=20.0)
+    ref = o.OracleSource(carrier_frequency_hz=20.0)
+    dt = 2.4e-6
+    times = (np.arange(5000) + 0.5) * dt * 37
+    got = src.currents_a(times, dt)
+    want = np.array([ref.current_a(t, dt) for t in times])
+    # near carrier zero crossings the cosine's relative error grows; bound by the peak
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-14 * src.peak_current_a)
+    assert src.current_a(0.1, 1e-6) == 1.0e6  # peak at the default centre (4 half widths)
+
+
+def test_surface_impedance_tables_match_oracle():
+    model = g.ConductiveHalfSpaceSurface(1e-2)
+    t = surface_ade_tables(model, 480, 2.4e-6)
+    alpha = 0.5 * 2.4e-6 * o.si_pole_rates()
+    np.testing.assert_array_equal(model.pole_rates_s_inv, o.si_pole_rates())
+    np.testing.assert_array_equal(model.diffusive_weights_sqrt_s_inv, o.si_weights())
+    np.testing.assert_array_equal(t["decay"], (1 - alpha) / (1 + alpha))
+    np.testing.assert_array_equal(t["history_weights"], o.si_weights() / (1 + alpha))
+    # positive-real fit within the reference's 6e-3 of sqrt(mu s / sigma) over the band
+    f = np.linspace(5.0, 45.0, 41)
+    fit = model.impedance_ohm(f)[0]
+    exact = model.exact_impedance_ohm(f)[0]
+    assert np.max(np.abs(fit - exact) / np.abs(exact)) < 6e-3
+
+
+def test_plasma_tables_match_oracle_coupler():
+    sc = S.scenario("magnetized")
+    mesh = g.build_geodesic_mesh(sc["subdivision"])
+    alt = S._midpoint_altitudes(sc)
+    b, species = S._plasma_arrays(sc, mesh, alt)
+    model = g.MeshPlasmaModel.from_mesh(mesh, alt, b, tuple(g.ColdPlasmaSpecies(*s)
+                                                            for s in species))
+    dt = 1.6e-7
+    t = plasma_tables(model, mesh, dt)
+    om = o.build_mesh(sc["subdivision"])
+    coupler = o.OraclePlasmaCoupler(o.OraclePlasma(b, [o.OracleSpecies(q, m, n, nu)
+                                                       for _, q, m, n, nu in species]), om, dt)
+    np.testing.assert_array_equal(t["reconstruction"], coupler.reconstruction)
+    np.testing.assert_array_equal(t["left_tangents"], coupler.left_tangents)
+    np.testing.assert_array_equal(t["right_tangents"], coupler.right_tangents)
+    assert np.array_equal(t["vertex_faces"], coupler.vertex_faces)
+    np.testing.assert_array_equal(t["vertex_face_weights"], coupler.vertex_face_weights)
+    rows = t["coef"].shape[2]
+    for s, coefs in enumerate(coupler.coefficients):
+        for q, c in enumerate(coefs):
+            want = c if rows > 1 else c[:1]
+            np.testing.assert_array_equal(t["coef"][s, q], want)
+    assert t["magnetized"]
+    assert t["band"] == (int(np.flatnonzero(alt >= 60e3)[0]), len(alt))
+
+
+def test_pinv_reconstruction_recovers_tangential_vectors():
+    """plasma.py reconstruction: any tangent vector is recovered from its three
+    edge-tangent projections (test_plasma.py:98-108 tolerance 8e-16 scale)."""
+    mesh = g.build_geodesic_mesh(2)
+    model = g.electron_drude_model(mesh, np.array([80e3]), 1e9, 1e5)
+    t = plasma_tables(model, mesh, 1e-7)
+    rng = np.random.default_rng(3)
+    v = rng.standard_normal((mesh.n_faces, 3))
+    v -= np.einsum("ij,ij->i", v, mesh.face_centers)[:, None] * mesh.face_centers
+    proj = np.einsum("fcd,fd->fc", t["face_tangents"], v)
+    back = np.einsum("fc,fcd->fd", proj, t["reconstruction"])
+    np.testing.assert_allclose(back, v, atol=8e-15)
+
+
+def test_config_validation_mirrors_reference():
+    with pytest.raises(ValueError, match="radial_cells"):
+        g.SimulationConfig(radial_cells=1)
+    with pytest.raises(ValueError, match="surface-impedance"):
+        g.SimulationConfig(radial_boundary_condition="surface-impedance")
+    with pytest.raises(ValueError, match="courant"):
+        g.SimulationConfig(courant_factor=1.5)
+    with pytest.raises(ValueError, match="integers"):
+        g.SimulationConfig(subdivision=2.0)
+    with pytest.raises(ValueError, match="reversed"):
+        g.SimulationConfig(minimum_altitude_m=10.0, maximum_altitude_m=0.0)
+    with pytest.raises(ValueError, match="geometry_mode"):
+        g.SimulationConfig(geometry_mode="flat")
+    g.SimulationConfig(radial_cells=3, minimum_altitude_m=0.0, maximum_altitude_m=3.0,
+                       radial_altitudes_m=(0.0, 1.0, 2.0, 3.0))
+
+
+def test_library_exports_every_declared_symbol():
+    header = (REPO / "include" / "geofdtd.h").read_text()
+    declared = set(re.findall(r"\b(geo_[a-z_]+)\s*\(", header))
+    assert declared == set(_native.EXPORTED_SYMBOLS)
+    lib = _native.load_library()
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.geo_version() == 1
+
+
+def test_product_refuses_cpu_backends_and_missing_device(monkeypatch):
+    with pytest.raises(g.BackendUnavailableError):
+        g.create_backend("numpy", g.build_geodesic_mesh(0))
+    with pytest.raises(ValueError):
+        g.create_backend("jax", g.build_geodesic_mesh(0))
+    import torch
+
+    if not torch.cuda.is_available():
+        with pytest.raises(g.BackendUnavailableError):
+            g.GeodesicFDTD(g.SimulationConfig(subdivision=0, radial_cells=2))
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    with pytest.raises(g.BackendUnavailableError, match="not built"):
+        _native.load_library(tmp_path / "libgeofdtd.so")
+
+
+def test_radial_refinement_matches_reference_rules():
+    from ionosphere_fdtd_amd.radial_grid import RadialRefinementRegion
+
+    nodes = g.build_refined_radial_grid(0.0, 100e3, 2e3,
+                                        (RadialRefinementRegion(60e3, 90e3, 500.0),))
+    steps = np.diff(nodes)
+    assert np.isclose(steps.min(), 500.0) and np.isclose(steps.max(), 2000.0)
+    assert np.max(np.maximum(steps[:-1] / steps[1:], steps[1:] / steps[:-1])) <= 2.0
