@@ -111,6 +111,12 @@ def main() -> int:
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--kernel-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="c1",
+                    help="c1 (default), c1_noplasma, c1_pec, vacuum_s7_r64, vacuum_s7_r80")
+    ap.add_argument("--stream-ceiling", action="store_true",
+                    help="also time a 1 GiB device copy (read + write GB/s)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run 4 torch copies of 512 MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the multi-GPU halo path is out of scope for this "
@@ -122,9 +128,11 @@ def main() -> int:
     import scenarios as S
 
     torch.cuda.set_device(0)
-    sc = S.scenario("c1")
+    sc = S.scenario(args.workload)
     t_setup = time.perf_counter()
     sim = S.build_sim(sc, compile_step=True, compile_chunk_size=args.chunk)
+    if sc["seed_fields"]:
+        S.apply_seed_fields(sim, lambda k, v: setattr(sim, k, v))
     pts = S.receiver_points(sc)
     v, l, w = S.receiver_tables(sim.mesh, sim.altitudes_m, pts)
     setup_s = time.perf_counter() - t_setup
@@ -176,7 +184,24 @@ def main() -> int:
                      "kernel_ms": {"h_step_kernel": round(h_ms, 5), "e_step_kernel": round(e_ms, 5)},
                      "step_fraction": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
     }
-    if not args.no_cpu_baseline:
+    if args.workload != "c1":
+        result["config"]["workload"] = args.workload
+    if args.stream_ceiling:
+        from ionosphere_fdtd_amd import _native
+
+        probe = {}
+        for mode, label in ((0, "copy"), (1, "read"), (2, "write"), (4, "copy_nt"),
+                            (5, "read_nt"), (6, "write_nt")):
+            probe[label] = round(_native.stream_probe(torch, 1 << 30, mode), 1)
+        result["stream_probe_gbs"] = probe
+    if args.copy_calibration:
+        # four launches of the library's probe copy kernel, 512 MiB each way:
+        # their FETCH_SIZE / WRITE_SIZE readings calibrate the counters
+        from ionosphere_fdtd_amd import _native
+
+        _native.stream_probe(torch, 512 << 20, 0, repeats=3)
+        result["copy_calibration_bytes_each_way"] = 512 << 20
+    if not args.no_cpu_baseline and args.workload == "c1":
         result["cpu_baseline"] = cpu_baseline()
     print(json.dumps(result))
     return 0

@@ -136,6 +136,7 @@ typedef struct GeoPhysicsDesc {
   int32_t magnetized;                /* 0: B == 0 everywhere (Drude form) */
   int32_t band_lo, band_hi;          /* midpoint layers with plasma state */
   const double* plasma_reconstruction; /* (F,3,3) pinv(tangents).T */
+  const double* face_tangents;       /* (F,3,3) unit edge tangents of each face */
   const double* face_centers;        /* (F,3) */
   const double* left_tangents;       /* (E,3) */
   const double* right_tangents;      /* (E,3) */
@@ -156,8 +157,8 @@ typedef struct GeoState {
   void* ht; int64_t ht_pitch;        /* (E, ht_pitch) */
   void* et; int64_t et_pitch;        /* (E, et_pitch) */
   void* hr; int64_t hr_pitch;        /* (F, hr_pitch) */
-  void* si_memory;                   /* (E, si_terms), compute type */
-  void* plasma_current;              /* (n_species, 3, F, plasma_pitch), compute type */
+  void* si_memory;                   /* (si_terms, E) term-major, compute type */
+  void* plasma_current;              /* (n_species, components, F, plasma_pitch), compute */
   int64_t plasma_pitch;
 } GeoState;
 
@@ -168,6 +169,8 @@ typedef struct GeoLayout {
   int64_t band_lo;         /* first plasma midpoint layer stored */
   int64_t band_hi;         /* one past the last */
   int64_t plasma_pitch;    /* pitch of plasma rows (>= band_hi - band_lo) */
+  int64_t plasma_components; /* 4 = (J.n, J.t0, J.t1, J.t2) per face (B = 0),
+                                3 = Cartesian J (magnetised) */
   int64_t vector_elems;    /* elements per 16-byte strip */
   int64_t compute_bytes;   /* 4 (FP32 arithmetic) or 8 (FP64) */
 } GeoLayout;
@@ -203,6 +206,13 @@ int geo_diagnostics(GeoPlan* plan, GeoDiagOut* out, void* stream);
  * event-timed steps: out_ms[0] = H kernel, out_ms[1] = E kernel. */
 int geo_time_kernels(GeoPlan* plan, int64_t n_steps, void* stream, double* out_ms);
 void geo_plan_destroy(GeoPlan* plan);
+/* Bandwidth probe (no counterpart in the reference): average time (ms) of a
+ * 16-byte-strip kernel over `bytes` -- mode 0 copy src->dst, 1 read src,
+ * 2 write dst, +4 with non-temporal hints.  The measured copy rate is the
+ * achievable HBM ceiling reported beside the roofline, and its rocprofv3
+ * FETCH_SIZE / WRITE_SIZE readings calibrate the counters. */
+int geo_stream_probe(void* dst, const void* src, int64_t bytes, int32_t mode, int32_t repeats,
+                     void* stream, double* out_ms);
 const char* geo_last_error(void);
 int geo_version(void);
 

@@ -61,7 +61,7 @@ class GeoPhysicsDesc(C.Structure):
         ("tsource_edges", _p_i64), ("tsource_layers", _p_i64), ("tsource_weights", _p_f64),
         ("n_species", C.c_int32), ("magnetized", C.c_int32),
         ("band_lo", C.c_int32), ("band_hi", C.c_int32),
-        ("plasma_reconstruction", _p_f64), ("face_centers", _p_f64),
+        ("plasma_reconstruction", _p_f64), ("face_tangents", _p_f64), ("face_centers", _p_f64),
         ("left_tangents", _p_f64), ("right_tangents", _p_f64),
         ("vertex_faces", _p_i64), ("vertex_face_weights", _p_f64), ("vf_width", C.c_int64),
         ("plasma_coef", _p_f64), ("plasma_coef_rows", C.c_int64),
@@ -80,6 +80,7 @@ class GeoState(C.Structure):
 class GeoLayout(C.Structure):
     _fields_ = [("node_pitch", C.c_int64), ("mid_pitch", C.c_int64), ("band_lo", C.c_int64),
                 ("band_hi", C.c_int64), ("plasma_pitch", C.c_int64),
+                ("plasma_components", C.c_int64),
                 ("vector_elems", C.c_int64), ("compute_bytes", C.c_int64)]
 
 
@@ -94,7 +95,7 @@ class GeoDiagOut(C.Structure):
 EXPORTED_SYMBOLS = (
     "geo_plan_create", "geo_plan_layout", "geo_plan_bind", "geo_step", "geo_set_receivers",
     "geo_sample_er", "geo_diagnostics", "geo_time_kernels", "geo_plan_destroy",
-    "geo_last_error", "geo_version",
+    "geo_last_error", "geo_version", "geo_stream_probe",
 )
 
 _lib = None
@@ -120,6 +121,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.geo_sample_er.argtypes = [vp, vp, vp]
     lib.geo_diagnostics.argtypes = [vp, C.POINTER(GeoDiagOut), vp]
     lib.geo_time_kernels.argtypes = [vp, C.c_int64, vp, _p_f64]
+    lib.geo_stream_probe.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, _p_f64]
     lib.geo_plan_destroy.argtypes = [vp]
     lib.geo_plan_destroy.restype = None
     lib.geo_last_error.restype = C.c_char_p
@@ -131,6 +133,20 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     if path is None:
         _lib = lib
     return lib
+
+
+def stream_probe(torch, bytes_: int, mode: int, repeats: int = 10) -> float:
+    """Achieved GB/s of the library's 16-byte-strip probe kernel (see the header)."""
+    lib = load_library()
+    n = bytes_ // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    out = C.c_double()
+    check(lib.geo_stream_probe(dst.data_ptr(), src.data_ptr(), n * 4, mode, repeats,
+                               C.c_void_p(torch.cuda.current_stream().cuda_stream),
+                               C.byref(out)), lib)
+    moved = (2 if (mode & 3) == 0 else 1) * n * 4
+    return moved / (out.value * 1e-3) / 1e9
 
 
 def check(code: int, lib: C.CDLL | None = None) -> None:

@@ -312,8 +312,11 @@ struct Plan final : PlanBase {
             (C)(dtm * (double)m->face_edge_signs[3 * f + c] * m->primal_edge_angles[e] / omega);
       }
     }
-    std::vector<int32_t> vedges(6 * V, 0);
-    std::vector<C> vw(6 * V, 0);
+    // Dual-cell incidence padded to 8 slots (two 16-byte loads per table row);
+    // padding slots repeat slot 0's edge with weight 0.
+    std::vector<int32_t> vedges(8 * V, 0);
+    std::vector<C> vw(8 * V, 0);
+    std::vector<int32_t> min_edge(V, (int32_t)E);
     for (int64_t v = 0; v < V; ++v) {
       const double area = m->dual_cell_solid_angles[v];
       if (!(area > 0.0)) return fail(GEO_ERR_INVALID, "degenerate dual cell");
@@ -321,11 +324,19 @@ struct Plan final : PlanBase {
         const int64_t e = m->vertex_edges[v * m->max_degree + s];
         const int64_t sg = m->vertex_edge_signs[v * m->max_degree + s];
         if (e < 0 || e >= E) return fail(GEO_ERR_INVALID, "vertex incidence out of range");
-        vedges[6 * v + s] = (int32_t)e;
-        vw[6 * v + s] = (C)((double)sg * m->dual_edge_angles[e] / area);
+        vedges[8 * v + s] = (int32_t)e;
+        vw[8 * v + s] = (C)((double)sg * m->dual_edge_angles[e] / area);
+        if (sg != 0) min_edge[v] = std::min(min_edge[v], (int32_t)e);
       }
-      for (int64_t s = m->max_degree; s < 6; ++s) vedges[6 * v + s] = vedges[6 * v];
+      for (int64_t s = m->max_degree; s < 8; ++s) vedges[8 * v + s] = vedges[8 * v];
     }
+    // E-kernel processing order of the dual cells: by first incident edge, so
+    // the Ht columns a vertex gathers are the ones the edge segment streams at
+    // the same time (storage order stays the reference's).
+    std::vector<int32_t> vorder(V);
+    for (int64_t v = 0; v < V; ++v) vorder[v] = (int32_t)v;
+    std::stable_sort(vorder.begin(), vorder.end(),
+                     [&](int32_t a, int32_t b) { return min_edge[a] < min_edge[b]; });
     // ---- loss coefficients ----
     const int64_t er_rows = ph->er_coef_rows, et_rows = ph->et_coef_rows;
     if (!(er_rows == 1 || er_rows == V) || !(et_rows == 1 || et_rows == E))
@@ -365,7 +376,8 @@ struct Plan final : PlanBase {
     magnetized = plasma && ph->magnetized != 0;
     int64_t band_lo = 0, band_hi = 0, ppitch = 0;
     std::vector<C> recon, center, ltan, rtan, vfw, coef, bdir;
-    std::vector<int32_t> vfaces;
+    std::vector<int32_t> vfaces, eslots;
+    int64_t ncomp = 0;
     if (plasma) {
       if (ph->n_species > geo::kMaxSpecies) return fail(GEO_ERR_INVALID, "too many species");
       if (ph->band_lo < 0 || ph->band_hi > R || ph->band_lo >= ph->band_hi)
@@ -373,20 +385,57 @@ struct Plan final : PlanBase {
       band_lo = ph->band_lo / N * N;
       band_hi = ph->band_hi;
       ppitch = round_up(band_hi - band_lo, N);
-      recon.assign(ph->plasma_reconstruction, ph->plasma_reconstruction + 9 * F);
-      center.assign(ph->face_centers, ph->face_centers + 3 * F);
-      ltan.assign(ph->left_tangents, ph->left_tangents + 3 * E);
-      rtan.assign(ph->right_tangents, ph->right_tangents + 3 * E);
+      ncomp = magnetized ? 3 : 4;
+      if (magnetized) {
+        recon.assign(ph->plasma_reconstruction, ph->plasma_reconstruction + 9 * F);
+      } else {
+        // Projected state for B = 0: the ADE is linear and isotropic, so the
+        // projections the scatter needs (J.n, J.t_c) obey the same update as
+        // J with E.n = radial average and E.t_c = sum_c' P[c][c'] Et_c',
+        // P[c][c'] = t_c . R_c' (R = pinv(T).T, plasma.py:548-559).
+        if (!ph->face_tangents) return fail(GEO_ERR_INVALID, "face tangents are required");
+        recon.assign(9 * F, 0);
+        for (int64_t f = 0; f < F; ++f)
+          for (int c = 0; c < 3; ++c)
+            for (int cp = 0; cp < 3; ++cp) {
+              double dot = 0.0;
+              for (int d = 0; d < 3; ++d)
+                dot += ph->face_tangents[(f * 3 + c) * 3 + d] *
+                       ph->plasma_reconstruction[(f * 3 + cp) * 3 + d];
+              recon[9 * f + 3 * c + cp] = (C)dot;
+            }
+        // local slot of every edge in its left and right face
+        eslots.assign(2 * E, 0);
+        for (int64_t e = 0; e < E; ++e)
+          for (int side = 0; side < 2; ++side) {
+            const int64_t f = side == 0 ? m->edge_left_faces[e] : m->edge_right_faces[e];
+            int slot = -1;
+            for (int c = 0; c < 3; ++c)
+              if (m->face_edges[3 * f + c] == e) slot = c;
+            if (slot < 0) return fail(GEO_ERR_INVALID, "edge is absent from an adjacent face");
+            eslots[2 * e + side] = slot;
+          }
+      }
+      center.assign(4 * F, 0);
+      for (int64_t f = 0; f < F; ++f)
+        for (int d = 0; d < 3; ++d) center[4 * f + d] = (C)ph->face_centers[3 * f + d];
+      ltan.assign(4 * E, 0);
+      rtan.assign(4 * E, 0);
+      for (int64_t e = 0; e < E; ++e)
+        for (int d = 0; d < 3; ++d) {
+          ltan[4 * e + d] = (C)ph->left_tangents[3 * e + d];
+          rtan[4 * e + d] = (C)ph->right_tangents[3 * e + d];
+        }
       if (ph->vf_width < 1 || ph->vf_width > 6)
         return fail(GEO_ERR_INVALID, "vertex-face width must be at most 6");
-      vfaces.assign(6 * V, 0);
-      vfw.assign(6 * V, 0);
+      vfaces.assign(8 * V, 0);
+      vfw.assign(8 * V, 0);
       for (int64_t v = 0; v < V; ++v)
         for (int64_t s = 0; s < ph->vf_width; ++s) {
           const int64_t f = ph->vertex_faces[v * ph->vf_width + s];
           if (f < 0 || f >= F) return fail(GEO_ERR_INVALID, "vertex face out of range");
-          vfaces[6 * v + s] = (int32_t)f;
-          vfw[6 * v + s] = (C)ph->vertex_face_weights[v * ph->vf_width + s];
+          vfaces[8 * v + s] = (int32_t)f;
+          vfw[8 * v + s] = (C)ph->vertex_face_weights[v * ph->vf_width + s];
         }
       const int64_t rows = ph->plasma_coef_rows;
       if (!(rows == 1 || rows == F)) return fail(GEO_ERR_INVALID, "plasma rows must be 1 or F");
@@ -411,13 +460,14 @@ struct Plan final : PlanBase {
     int rc;
 #define UP(vec, ptr) \
   if ((rc = arena.upload(vec, &ptr)) != GEO_OK) return rc
-    int32_t *d_edges, *d_lr, *d_fe, *d_faces, *d_ve, *d_vf = nullptr;
+    int32_t *d_edges, *d_lr, *d_fe, *d_faces, *d_ve, *d_vo, *d_vf = nullptr;
     C *d_eg, *d_ed, *d_fc, *d_vw, *d_ir, *d_rdb, *d_rdc, *d_irb, *d_rha, *d_rhc;
     C *d_caer, *d_cber, *d_caet, *d_cbet, *d_sis = nullptr;
     C *d_recon = nullptr, *d_center = nullptr, *d_lt = nullptr, *d_rt = nullptr;
     C *d_vfw = nullptr, *d_coef = nullptr, *d_b = nullptr;
+    int32_t* d_es = nullptr;
     UP(edges, d_edges); UP(edge_lr, d_lr); UP(face_edges, d_fe); UP(faces, d_faces);
-    UP(vedges, d_ve); UP(edge_grad, d_eg); UP(edge_dual, d_ed); UP(face_curl, d_fc);
+    UP(vedges, d_ve); UP(vorder, d_vo); UP(edge_grad, d_eg); UP(edge_dual, d_ed); UP(face_curl, d_fc);
     UP(vw, d_vw); UP(inv_r, d_ir); UP(rd_b, d_rdb); UP(rd_c, d_rdc); UP(inv_rbar, d_irb);
     UP(rh_a, d_rha); UP(rh_c, d_rhc); UP(ca_er, d_caer); UP(cb_er, d_cber);
     UP(ca_et, d_caet); UP(cb_et, d_cbet);
@@ -425,7 +475,11 @@ struct Plan final : PlanBase {
     if (plasma) {
       UP(recon, d_recon); UP(center, d_center); UP(ltan, d_lt); UP(rtan, d_rt);
       UP(vfaces, d_vf); UP(vfw, d_vfw); UP(coef, d_coef);
-      if (magnetized) UP(bdir, d_b);
+      if (magnetized) {
+        UP(bdir, d_b);
+      } else {
+        UP(eslots, d_es);
+      }
     }
 #undef UP
     void* dyn_mem = nullptr;
@@ -439,7 +493,8 @@ struct Plan final : PlanBase {
       pl.band_hi = (int32_t)band_hi;
       pl.pitch = (int32_t)ppitch;
       pl.comp_stride = F * ppitch;
-      pl.species_stride = 3 * F * ppitch;
+      pl.species_stride = ncomp * F * ppitch;
+      pl.edge_slots = d_es;
       pl.recon = d_recon;
       pl.center = d_center;
       pl.left_tan = d_lt;
@@ -520,6 +575,7 @@ struct Plan final : PlanBase {
     ep.s_et_inv = (C)(1.0 / s_et);
     ep.s_ht_inv = (C)(1.0 / s_ht);
     ep.s_hr_inv = (C)(1.0 / s_hr);
+    ep.vertex_order = d_vo;
     ep.vertex_edges = d_ve;
     ep.vertex_w = d_vw;
     ep.edge_lr = d_lr;
@@ -606,6 +662,7 @@ struct Plan final : PlanBase {
     layout.band_lo = band_lo;
     layout.band_hi = band_hi;
     layout.plasma_pitch = ppitch;
+    layout.plasma_components = ncomp;
     layout.vector_elems = N;
     layout.compute_bytes = sizeof(C);
     GEO_HIP(hipStreamCreateWithFlags(&capture_stream, hipStreamNonBlocking));
@@ -685,8 +742,12 @@ struct Plan final : PlanBase {
     }
   }
   void launch_e(hipStream_t s) {
-    if (plasma) hipLaunchKernelGGL((geo::e_step_kernel<T, C, true>), dim3(e_grid), dim3(geo::kBlock), 0, s, ep);
-    else hipLaunchKernelGGL((geo::e_step_kernel<T, C, false>), dim3(e_grid), dim3(geo::kBlock), 0, s, ep);
+    if (plasma) {
+      if (magnetized) hipLaunchKernelGGL((geo::e_step_kernel<T, C, true, true>), dim3(e_grid), dim3(geo::kBlock), 0, s, ep);
+      else hipLaunchKernelGGL((geo::e_step_kernel<T, C, true, false>), dim3(e_grid), dim3(geo::kBlock), 0, s, ep);
+    } else {
+      hipLaunchKernelGGL((geo::e_step_kernel<T, C, false, false>), dim3(e_grid), dim3(geo::kBlock), 0, s, ep);
+    }
   }
 
   int ensure_graph(int32_t chunk) {
@@ -818,6 +879,39 @@ struct Plan final : PlanBase {
   }
 };
 
+// Bandwidth probe: 16-byte strips, grid-stride.  mode 0 copy, 1 read (sum
+// into a per-thread register written only when the impossible sentinel
+// matches, so the loads are not removed), 2 write; +4 = non-temporal hints.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(geo::kBlock) stream_probe_kernel(f4v* __restrict__ dst,
+                                                                   const f4v* __restrict__ src,
+                                                                   int64_t n, int32_t mode) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  float acc = 0.f;
+  const bool nt = (mode & 4) != 0;
+  const int32_t kind = mode & 3;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (kind == 2) {
+      const f4v v = {1.f, 2.f, 3.f, (float)i};
+      if (nt) __builtin_nontemporal_store(v, dst + i);
+      else dst[i] = v;
+    } else {
+      const f4v v = nt ? __builtin_nontemporal_load(src + i) : src[i];
+      if (kind == 0) {
+        if (nt) __builtin_nontemporal_store(v, dst + i);
+        else dst[i] = v;
+      } else {
+        acc += v.x + v.y + v.z + v.w;
+      }
+    }
+  }
+  if (kind == 1 && acc == -1.2345e-38f) {
+    const f4v z = {acc, acc, acc, acc};
+    dst[0] = z;
+  }
+}
+
 int check_device() {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
@@ -925,5 +1019,35 @@ int geo_time_kernels(GeoPlan* plan, int64_t n_steps, void* stream, double* out_m
 }
 
 void geo_plan_destroy(GeoPlan* plan) { delete plan; }
+
+int geo_stream_probe(void* dst, const void* src, int64_t bytes, int32_t mode, int32_t repeats,
+                     void* stream, double* out_ms) {
+  if (!dst || bytes <= 0 || (bytes & 15) || repeats < 1 || !out_ms)
+    return fail(GEO_ERR_INVALID, "stream probe needs a 16-byte multiple and a destination");
+  if ((mode & 3) != 2 && !src) return fail(GEO_ERR_INVALID, "stream probe needs a source");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  int dev = 0, cus = 0;
+  GEO_HIP(hipGetDevice(&dev));
+  GEO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t n = bytes / 16;
+  const int grid = (int)std::min<int64_t>((n + geo::kBlock - 1) / geo::kBlock, (int64_t)cus * 64);
+  hipEvent_t a, b;
+  GEO_HIP(hipEventCreate(&a));
+  GEO_HIP(hipEventCreate(&b));
+  hipLaunchKernelGGL(stream_probe_kernel, dim3(grid), dim3(geo::kBlock), 0, s,
+                     static_cast<f4v*>(dst), static_cast<const f4v*>(src), n, mode);
+  GEO_HIP(hipEventRecord(a, s));
+  for (int r = 0; r < repeats; ++r)
+    hipLaunchKernelGGL(stream_probe_kernel, dim3(grid), dim3(geo::kBlock), 0, s,
+                       static_cast<f4v*>(dst), static_cast<const f4v*>(src), n, mode);
+  GEO_HIP(hipEventRecord(b, s));
+  GEO_HIP(hipEventSynchronize(b));
+  float ms = 0.f;
+  GEO_HIP(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  *out_ms = ms / repeats;
+  return GEO_OK;
+}
 
 }  // extern "C"

@@ -30,6 +30,7 @@ namespace geo {
 constexpr int kBlock = 256;
 constexpr int kMaxSource = 16;
 constexpr int kMaxSiTerms = 32;
+constexpr int kSiFastTerms = 16;  // terms held in registers (the reference default)
 constexpr int kMaxSpecies = 4;
 constexpr int kNumXcd = 8;
 
@@ -116,9 +117,33 @@ __device__ __forceinline__ void store_one(T* __restrict__ p, C x, C scale) {
   else *p = from_c<T>(x);
 }
 
+// Aligned loads / stores of M compute-type values (16-byte multiples): the
+// radial tables, loss coefficients and plasma arrays share the strip
+// alignment of the fields, so one vector access replaces M lane loads.
+template <typename C, int M>
+struct alignas(16) CVec {
+  C v[M];
+};
+
+template <int M, typename C>
+__device__ __forceinline__ void load_vec(const C* __restrict__ p, C* out) {
+  const CVec<C, M> x = *reinterpret_cast<const CVec<C, M>*>(p);
+#pragma unroll
+  for (int i = 0; i < M; ++i) out[i] = x.v[i];
+}
+
+template <int M, typename C>
+__device__ __forceinline__ void store_vec(C* __restrict__ p, const C* in) {
+  CVec<C, M> x;
+#pragma unroll
+  for (int i = 0; i < M; ++i) x.v[i] = in[i];
+  *reinterpret_cast<CVec<C, M>*>(p) = x;
+}
+
 // Logical block index: hardware blocks are dispatched round-robin over the 8
 // XCDs, so block b runs on XCD b % 8.  Give XCD x the contiguous logical range
-// [x * per_xcd, (x + 1) * per_xcd) so neighbour columns stay in its L2.
+// [x * per_xcd, (x + 1) * per_xcd) of the block map so neighbour columns stay
+// in its L2 and blocks running together on one XCD work on adjacent regions.
 __device__ __forceinline__ int32_t logical_block(int32_t per_xcd) {
   const int32_t b = blockIdx.x;
   return (b % kNumXcd) * per_xcd + b / kNumXcd;
@@ -139,15 +164,18 @@ struct PlasmaTables {
   int32_t n_species;
   int32_t band_lo, band_hi;     // band_lo is a multiple of the strip width
   int32_t pitch;                // J row pitch (compute elements)
-  int64_t species_stride;       // elements between species blocks (3 * F * pitch)
-  int64_t comp_stride;          // elements between x/y/z blocks (F * pitch)
-  C* current;                   // (species, 3, F, pitch)
-  const C* recon;               // (F, 9)
-  const C* center;              // (F, 3)
-  const C* left_tan;            // (E, 3)
-  const C* right_tan;           // (E, 3)
-  const int32_t* vertex_faces;  // (V, 6)
-  const C* vertex_face_w;       // (V, 6)
+  int64_t species_stride;       // elements between species blocks (comps * F * pitch)
+  int64_t comp_stride;          // elements between component blocks (F * pitch)
+  // B = 0: (species, 4, F, pitch) holding J.n, J.t0, J.t1, J.t2;
+  // magnetised: (species, 3, F, pitch) Cartesian J.
+  C* current;
+  const C* recon;               // (F, 9): B = 0 P[c][c'] = t_c.R_c'; else R = pinv(T).T
+  const int32_t* edge_slots;    // (E, 2) slot of the edge in its left / right face
+  const C* center;              // (F, 4) unit normal, padded
+  const C* left_tan;            // (E, 4) padded
+  const C* right_tan;           // (E, 4) padded
+  const int32_t* vertex_faces;  // (V, 8) padded
+  const C* vertex_face_w;       // (V, 8) 0 on padding slots
   const C* coef;                // (species, 6, rows, pitch)
   int64_t coef_row_stride;      // 0 (uniform profile) or pitch
   int64_t coef_q_stride;        // elements between coefficient kinds
@@ -177,6 +205,7 @@ struct HParams {
   const C* __restrict__ inv_rbar;         // (pm,)
   // surface impedance
   C* __restrict__ si_mem;
+  // memory stored term-major, (terms, E), so every term access is coalesced
   const C* __restrict__ si_scale;         // (rows,) sqrt(mu/sigma)
   int32_t si_scale_stride;                // 0 or 1
   int32_t si_terms;
@@ -207,8 +236,9 @@ struct EParams {
   int32_t pn, pm, R;
   int32_t ca_er_stride, ca_et_stride;     // 0 or row pitch
   C s_er, s_er_inv, s_et, s_et_inv, s_ht_inv, s_hr_inv;
-  const int32_t* __restrict__ vertex_edges; // (V,6)
-  const C* __restrict__ vertex_w;           // (V,6) sign*phi/A
+  const int32_t* __restrict__ vertex_order;  // (V,) processing order
+  const int32_t* __restrict__ vertex_edges; // (V,8) padded
+  const C* __restrict__ vertex_w;           // (V,8) sign*phi/A, 0 on padding
   const int32_t* __restrict__ edge_lr;      // (E,2) left, right face
   const C* __restrict__ edge_dual;          // (E,) 1/phi
   const C* __restrict__ ca_er;
@@ -256,15 +286,18 @@ __device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, int32_t e, 
     for (int i = 0; i < N; ++i) et[i] = (C)0;
   }
   C prev = (k0 > 0) ? load_one(erow + k0 - 1, p.s_et_inv) : (C)0;
+  C rb[N], rc[N], ir[N];
+  load_vec<N>(p.rd_b + k0, rb);
+  load_vec<N>(p.rd_c + k0, rc);
+  load_vec<N>(p.inv_r + k0, ir);
   // Rows are padded with zeros and the radial tables are zero beyond R, so
   // the strip is updated unconditionally: Et(R) reads the zero padding (odd
   // ghost, PEC cap, solver.py:448-449) and padding lanes stay unchanged.
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    const int32_t k = k0 + i;
     const C cur = et[i];
-    const C rd = p.rd_b[k] * (cur - prev) + p.rd_c[k] * cur;
-    ht[i] = ht[i] + a * p.inv_r[k] * (erh[i] - ert[i]) - rd;
+    const C rd = rb[i] * (cur - prev) + rc[i] * cur;
+    ht[i] = ht[i] + a * ir[i] * (erh[i] - ert[i]) - rd;
     prev = cur;
   }
   if (p.si_on && k0 == 0) {
@@ -297,14 +330,18 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, int32_t f, 
   load_strip(p.et + (size_t)e2 * p.pm + j0, et2, p.s_et_inv);
   T* hrow = p.hr + (size_t)f * p.pm + j0;
   load_strip(hrow, hr, p.s_hr_inv);
+  C irb[N];
+  load_vec<N>(p.inv_rbar + j0, irb);
 #pragma unroll
   for (int i = 0; i < N; ++i)  // inv_rbar is zero on padding lanes
-    hr[i] = hr[i] - p.inv_rbar[j0 + i] * (w0 * et0[i] + w1 * et1[i] + w2 * et2[i]);
+    hr[i] = hr[i] - irb[i] * (w0 * et0[i] + w1 * et1[i] + w2 * et2[i]);
   store_strip(hrow, hr, p.s_hr);
 
   if constexpr (PLASMA) {
     const PlasmaTables<C>& pl = p.pl;
-    if (j0 + N <= pl.band_lo || j0 >= pl.band_hi) return;
+    // band_lo is a strip multiple, so a strip is either fully below the band
+    // or starts inside it; lanes at or beyond band_hi are J padding (zero).
+    if (j0 < pl.band_lo || j0 >= pl.band_hi) return;
     // E at (face, midpoint): tangential reconstruction + radial node average
     // (plasma.py:492-500).
     const int32_t v0 = p.faces[3 * f], v1 = p.faces[3 * f + 1], v2 = p.faces[3 * f + 2];
@@ -320,62 +357,88 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, int32_t f, 
     C rc[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) rc[q] = pl.recon[9 * f + q];
-    const C n0 = pl.center[3 * f], n1 = pl.center[3 * f + 1], n2 = pl.center[3 * f + 2];
+    C rad[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const C lower = (a0[i] + a1[i] + a2[i]) / (C)3;
+      const C upper = (a0[i + 1] + a1[i + 1] + a2[i + 1]) / (C)3;
+      rad[i] = (C)0.5 * (lower + upper);
+    }
     const int32_t jb0 = j0 - pl.band_lo;
-    for (int s = 0; s < pl.n_species; ++s) {
-      C* jx = pl.current + s * pl.species_stride + (size_t)f * pl.pitch + jb0;
-      C* jy = jx + pl.comp_stride;
-      C* jz = jy + pl.comp_stride;
-      const C* cf = pl.coef + s * pl.coef_species_stride + (size_t)f * pl.coef_row_stride + jb0;
+    if constexpr (!MAGNETIZED) {
+      // drive projections: E.n = rad, E.t_c = sum_c' P[c][c'] Et_c'
+      C drive[4][N];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
-        const int32_t j = j0 + i;
-        if (j < pl.band_lo || j >= pl.band_hi) continue;
-        const C lower = (a0[i] + a1[i] + a2[i]) / (C)3;
-        const C upper = (a0[i + 1] + a1[i + 1] + a2[i + 1]) / (C)3;
-        const C rad = (C)0.5 * (lower + upper);
-        C E[3];
+        drive[0][i] = rad[i];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          drive[1 + c][i] = (rc[3 * c] * et0[i] + rc[3 * c + 1] * et1[i]) + rc[3 * c + 2] * et2[i];
+      }
+      for (int s = 0; s < pl.n_species; ++s) {
+        C* jp = pl.current + s * pl.species_stride + (size_t)f * pl.pitch + jb0;
+        const C* cf = pl.coef + s * pl.coef_species_stride + (size_t)f * pl.coef_row_stride + jb0;
+        C decay[N], dre[N];
+        load_vec<N>(cf, decay);
+        load_vec<N>(cf + 4 * pl.coef_q_stride, dre);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          C x[N];
+          load_vec<N>(jp + q * pl.comp_stride, x);
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+            x[i] = (j0 + i < pl.band_hi) ? decay[i] * x[i] + dre[i] * drive[q][i] : x[i];
+          store_vec<N>(jp + q * pl.comp_stride, x);
+        }
+      }
+    } else {
+      C nrm[4];
+      load_vec<4>(pl.center + 4 * f, nrm);
+      C E[3][N];
+#pragma unroll
+      for (int i = 0; i < N; ++i)
 #pragma unroll
         for (int d = 0; d < 3; ++d)
-          E[d] = (et0[i] * rc[d] + et1[i] * rc[3 + d]) + et2[i] * rc[6 + d];
-        E[0] += rad * n0;
-        E[1] += rad * n1;
-        E[2] += rad * n2;
-        C J[3] = {jx[i], jy[i], jz[i]};
-        const C decay = cf[i];
-        C out[3];
-        if constexpr (!MAGNETIZED) {
-          const C dre = cf[4 * pl.coef_q_stride + i];
+          E[d][i] = (et0[i] * rc[d] + et1[i] * rc[3 + d]) + et2[i] * rc[6 + d] + rad[i] * nrm[d];
+      for (int s = 0; s < pl.n_species; ++s) {
+        C* jp = pl.current + s * pl.species_stride + (size_t)f * pl.pitch + jb0;
+        const C* cf = pl.coef + s * pl.coef_species_stride + (size_t)f * pl.coef_row_stride + jb0;
+        C J[3][N], decay[N], dre[N], cs[N], sn[N], dpar[N], dim[N];
 #pragma unroll
-          for (int d = 0; d < 3; ++d) out[d] = decay * J[d] + dre * E[d];
-        } else {
-          const C cs = cf[pl.coef_q_stride + i];
-          const C sn = cf[2 * pl.coef_q_stride + i];
-          const C dpar = cf[3 * pl.coef_q_stride + i];
-          const C dre = cf[4 * pl.coef_q_stride + i];
-          const C dim = cf[5 * pl.coef_q_stride + i];
+        for (int d = 0; d < 3; ++d) load_vec<N>(jp + d * pl.comp_stride, J[d]);
+        load_vec<N>(cf, decay);
+        load_vec<N>(cf + pl.coef_q_stride, cs);
+        load_vec<N>(cf + 2 * pl.coef_q_stride, sn);
+        load_vec<N>(cf + 3 * pl.coef_q_stride, dpar);
+        load_vec<N>(cf + 4 * pl.coef_q_stride, dre);
+        load_vec<N>(cf + 5 * pl.coef_q_stride, dim);
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          if (j0 + i >= pl.band_hi) continue;
           const C* bp = pl.bdir + (size_t)f * pl.b_row_stride + 3 * (jb0 + i);
           const C b[3] = {bp[0], bp[1], bp[2]};
-          const C es = (E[0] * b[0] + E[1] * b[1]) + E[2] * b[2];
-          const C js = (J[0] * b[0] + J[1] * b[1]) + J[2] * b[2];
+          const C Ei[3] = {E[0][i], E[1][i], E[2][i]};
+          const C Ji[3] = {J[0][i], J[1][i], J[2][i]};
+          const C es = (Ei[0] * b[0] + Ei[1] * b[1]) + Ei[2] * b[2];
+          const C js = (Ji[0] * b[0] + Ji[1] * b[1]) + Ji[2] * b[2];
           C epar[3], eperp[3], jpar[3], jperp[3], ecr[3], jcr[3];
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
             epar[d] = es * b[d];
-            eperp[d] = E[d] - epar[d];
+            eperp[d] = Ei[d] - epar[d];
             jpar[d] = js * b[d];
-            jperp[d] = J[d] - jpar[d];
+            jperp[d] = Ji[d] - jpar[d];
           }
           cross_b(eperp, b, ecr);
           cross_b(jperp, b, jcr);
 #pragma unroll
           for (int d = 0; d < 3; ++d)
-            out[d] = decay * jpar[d] + dpar * epar[d] + decay * (cs * jperp[d] + sn * jcr[d]) +
-                     dre * eperp[d] + dim * ecr[d];
+            J[d][i] = decay[i] * jpar[d] + dpar[i] * epar[d] +
+                      decay[i] * (cs[i] * jperp[d] + sn[i] * jcr[d]) + dre[i] * eperp[d] +
+                      dim[i] * ecr[d];
         }
-        jx[i] = out[0];
-        jy[i] = out[1];
-        jz[i] = out[2];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) store_vec<N>(jp + d * pl.comp_stride, J[d]);
       }
     }
   }
@@ -394,17 +457,38 @@ __device__ __forceinline__ void h_si_edge(const HParams<T, C>& p, int32_t e) {
   const C h_old = load_one(hp, p.s_ht_inv);
   const C et0 = load_one(p.et + (size_t)e * p.pm, p.s_et_inv);
   const C scale = p.si_scale[e * p.si_scale_stride];
-  C* mem = p.si_mem + (size_t)e * p.si_terms;
-  C hist = (C)0;
-  for (int i = 0; i < p.si_terms; ++i) hist += mem[i] * p.si_hw[i];
-  hist *= scale;
+  C* mem = p.si_mem + e;
+  const size_t ts = (size_t)p.n_edges;
   const C coupling = p.si_cs * (scale * p.si_hw_sum);
-  const C h_new = ((C)1 - (C)0.5 * coupling) * h_old + forcing - p.si_cs_metric * et0 +
-                  p.si_cs * hist;
-  const C h = h_new / ((C)1 + (C)0.5 * coupling);
-  store_one(hp, h, p.s_ht);
-  const C sum = h + h_old;
-  for (int i = 0; i < p.si_terms; ++i) mem[i] = mem[i] * p.si_decay[i] + sum * p.si_drive[i];
+  if (p.si_terms <= kSiFastTerms) {
+    // all terms in registers: the loads are independent and issue back to back
+    C m[kSiFastTerms];
+#pragma unroll
+    for (int i = 0; i < kSiFastTerms; ++i) m[i] = (i < p.si_terms) ? mem[i * ts] : (C)0;
+    C hist = (C)0;
+#pragma unroll
+    for (int i = 0; i < kSiFastTerms; ++i) hist += m[i] * p.si_hw[i];
+    hist *= scale;
+    const C h_new = ((C)1 - (C)0.5 * coupling) * h_old + forcing - p.si_cs_metric * et0 +
+                    p.si_cs * hist;
+    const C h = h_new / ((C)1 + (C)0.5 * coupling);
+    store_one(hp, h, p.s_ht);
+    const C sum = h + h_old;
+#pragma unroll
+    for (int i = 0; i < kSiFastTerms; ++i)
+      if (i < p.si_terms) mem[i * ts] = m[i] * p.si_decay[i] + sum * p.si_drive[i];
+  } else {
+    C hist = (C)0;
+    for (int i = 0; i < p.si_terms; ++i) hist += mem[i * ts] * p.si_hw[i];
+    hist *= scale;
+    const C h_new = ((C)1 - (C)0.5 * coupling) * h_old + forcing - p.si_cs_metric * et0 +
+                    p.si_cs * hist;
+    const C h = h_new / ((C)1 + (C)0.5 * coupling);
+    store_one(hp, h, p.s_ht);
+    const C sum = h + h_old;
+    for (int i = 0; i < p.si_terms; ++i)
+      mem[i * ts] = mem[i * ts] * p.si_decay[i] + sum * p.si_drive[i];
+  }
 }
 
 template <typename T, typename C>
@@ -458,62 +542,118 @@ __global__ void __launch_bounds__(kBlock) h_step_kernel(const HParams<T, C> p) {
 // E half-step
 // ---------------------------------------------------------------------------
 
-// Sum over species of J(face, j) . vec, reference accumulation order.
-template <typename C>
-__device__ __forceinline__ C plasma_dot(const PlasmaTables<C>& pl, int32_t f, int32_t j,
-                                        C v0, C v1, C v2) {
-  const int32_t jb = j - pl.band_lo;
-  C jt[3] = {(C)0, (C)0, (C)0};
+// Species-summed J(face, j0 + i) . v for the N lanes of a strip starting at
+// an in-band j0 (lanes past the band read the zero padding), plus the value
+// at j0 - 1 when that layer is in the band.
+// Species sum of one stored component (projected state) for the N lanes of
+// an in-band strip at j0, plus the value at j0 - 1 when that layer is in band.
+template <typename C, int N, bool WANT_PREV>
+__device__ __forceinline__ void plasma_component(const PlasmaTables<C>& pl, int32_t f,
+                                                 int32_t comp, int32_t j0, C* strip, C* prev) {
+  const bool has_strip = j0 >= pl.band_lo && j0 < pl.band_hi;
+  const bool has_prev = WANT_PREV && j0 - 1 >= pl.band_lo && j0 - 1 < pl.band_hi;
+#pragma unroll
+  for (int i = 0; i < N; ++i) strip[i] = (C)0;
+  C pv = (C)0;
+  const int32_t jb0 = j0 - pl.band_lo;
   for (int s = 0; s < pl.n_species; ++s) {
-    const C* base = pl.current + s * pl.species_stride + (size_t)f * pl.pitch + jb;
-    jt[0] += base[0];
-    jt[1] += base[pl.comp_stride];
-    jt[2] += base[2 * pl.comp_stride];
+    const C* base = pl.current + s * pl.species_stride + comp * pl.comp_stride + (size_t)f * pl.pitch;
+    if (has_strip) {
+      C x[N];
+      load_vec<N>(base + jb0, x);
+#pragma unroll
+      for (int i = 0; i < N; ++i) strip[i] += x[i];
+    }
+    if (has_prev) pv += base[jb0 - 1];
   }
-  return (jt[0] * v0 + jt[1] * v1) + jt[2] * v2;
+  if constexpr (WANT_PREV) *prev = pv;
 }
 
-template <typename T, typename C, bool PLASMA>
-__device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, int32_t v, int32_t strip) {
+template <typename C, int N, bool WANT_PREV>
+__device__ __forceinline__ void plasma_project(const PlasmaTables<C>& pl, int32_t f, int32_t j0,
+                                               const C* v, C* strip, C* prev) {
+  const bool has_strip = j0 >= pl.band_lo && j0 < pl.band_hi;
+  const bool has_prev = WANT_PREV && j0 - 1 >= pl.band_lo && j0 - 1 < pl.band_hi;
+  C jt[3][N];
+  C jp[3] = {(C)0, (C)0, (C)0};
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int i = 0; i < N; ++i) jt[d][i] = (C)0;
+  const int32_t jb0 = j0 - pl.band_lo;
+  for (int s = 0; s < pl.n_species; ++s) {
+    const C* base = pl.current + s * pl.species_stride + (size_t)f * pl.pitch;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (has_strip) {
+        C x[N];
+        load_vec<N>(base + d * pl.comp_stride + jb0, x);
+#pragma unroll
+        for (int i = 0; i < N; ++i) jt[d][i] += x[i];
+      }
+      if (has_prev) jp[d] += base[d * pl.comp_stride + jb0 - 1];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < N; ++i) strip[i] = (jt[0][i] * v[0] + jt[1][i] * v[1]) + jt[2][i] * v[2];
+  if constexpr (WANT_PREV) *prev = (jp[0] * v[0] + jp[1] * v[1]) + jp[2] * v[2];
+}
+
+template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
+__device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, int32_t row,
+                                               int32_t strip) {
   constexpr int N = Strip<T>::N;
+  const int32_t v = p.vertex_order[row];
   const int32_t k0 = strip * N;
+  int32_t ve[8];
+  C vw[8];
+  load_vec<4>(p.vertex_edges + 8 * v, ve);
+  load_vec<4>(p.vertex_edges + 8 * v + 4, ve + 4);
+  load_vec<8>(p.vertex_w + 8 * v, vw);
   C curl[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) curl[i] = (C)0;
 #pragma unroll
-  for (int s = 0; s < 6; ++s) {
-    const C w = p.vertex_w[6 * v + s];  // 0 on degree-5 padding slots
-    const int32_t e = p.vertex_edges[6 * v + s];
+  for (int s = 0; s < 6; ++s) {  // padding slots carry weight 0 and a valid edge
     C h[N];
-    load_strip(p.ht + (size_t)e * p.pn + k0, h, p.s_ht_inv);
+    load_strip(p.ht + (size_t)ve[s] * p.pn + k0, h, p.s_ht_inv);
 #pragma unroll
-    for (int i = 0; i < N; ++i) curl[i] += w * h[i];
+    for (int i = 0; i < N; ++i) curl[i] += vw[s] * h[i];
   }
   T* erow = p.er + (size_t)v * p.pn + k0;
-  C er[N];
+  C er[N], ca[N], cb[N], ir[N];
   load_strip(erow, er, p.s_er_inv);
-  const C* ca = p.ca_er + (size_t)v * p.ca_er_stride;
-  const C* cb = p.cb_er + (size_t)v * p.ca_er_stride;
+  load_vec<N>(p.ca_er + (size_t)v * p.ca_er_stride + k0, ca);
+  load_vec<N>(p.cb_er + (size_t)v * p.ca_er_stride + k0, cb);
+  load_vec<N>(p.inv_r + k0, ir);
   C jr[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) jr[i] = (C)0;
   if constexpr (PLASMA) {
     const PlasmaTables<C>& pl = p.pl;
-    // radial current at nodes k0-1 .. k0+N-1 needs midpoint layers k0-1 .. k0+N-1
-    if (k0 + N >= pl.band_lo && k0 - 1 < pl.band_hi) {
+    // nodes k0..k0+N-1 need midpoint layers k0-1..k0+N-1 (plasma.py:511-527)
+    if (k0 + N > pl.band_lo && k0 - 1 < pl.band_hi) {
       C rv[N + 1];
 #pragma unroll
       for (int i = 0; i <= N; ++i) rv[i] = (C)0;
+      int32_t vf[8];
+      C vfw[8];
+      load_vec<4>(pl.vertex_faces + 8 * v, vf);
+      load_vec<4>(pl.vertex_faces + 8 * v + 4, vf + 4);
+      load_vec<8>(pl.vertex_face_w + 8 * v, vfw);
       for (int s = 0; s < 6; ++s) {
-        const C w = pl.vertex_face_w[6 * v + s];
-        if (w == (C)0) continue;
-        const int32_t f = pl.vertex_faces[6 * v + s];
-        const C c0 = pl.center[3 * f], c1 = pl.center[3 * f + 1], c2 = pl.center[3 * f + 2];
-#pragma unroll
-        for (int i = 0; i <= N; ++i) {
-          const int32_t j = k0 - 1 + i;
-          if (j >= pl.band_lo && j < pl.band_hi) rv[i] += plasma_dot(pl, f, j, c0, c1, c2) * w;
+        if (vfw[s] == (C)0) continue;
+        C strip_v[N], prev;
+        if constexpr (MAGNETIZED) {
+          C nrm[4];
+          load_vec<4>(pl.center + 4 * vf[s], nrm);
+          plasma_project<C, N, true>(pl, vf[s], k0, nrm, strip_v, &prev);
+        } else {
+          plasma_component<C, N, true>(pl, vf[s], 0, k0, strip_v, &prev);
         }
+        rv[0] += prev * vfw[s];
+#pragma unroll
+        for (int i = 0; i < N; ++i) rv[i + 1] += strip_v[i] * vfw[s];
       }
 #pragma unroll
       for (int i = 0; i < N; ++i) {
@@ -539,14 +679,11 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, int32_t v
   }
   // ca/cb are zero on padding lanes, which therefore stay exactly zero.
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    const int32_t k = k0 + i;
-    er[i] = ca[k] * er[i] + cb[k] * (p.inv_r[k] * curl[i] - jr[i] - src[i]);
-  }
+  for (int i = 0; i < N; ++i) er[i] = ca[i] * er[i] + cb[i] * (ir[i] * curl[i] - jr[i] - src[i]);
   store_strip(erow, er, p.s_er);
 }
 
-template <typename T, typename C, bool PLASMA>
+template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
 __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, int32_t e, int32_t strip) {
   constexpr int N = Strip<T>::N;
   const int32_t j0 = strip * N;
@@ -561,22 +698,32 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, int32_t e, 
   ht[N] = (j0 + N <= p.R) ? load_one(hrow + j0 + N, p.s_ht_inv) : (C)0;
   T* erow = p.et + (size_t)e * p.pm + j0;
   load_strip(erow, et, p.s_et_inv);
-  const C* ca = p.ca_et + (size_t)e * p.ca_et_stride;
-  const C* cb = p.cb_et + (size_t)e * p.ca_et_stride;
+  C ca[N], cb[N], irb[N], ra[N], rcc[N];
+  load_vec<N>(p.ca_et + (size_t)e * p.ca_et_stride + j0, ca);
+  load_vec<N>(p.cb_et + (size_t)e * p.ca_et_stride + j0, cb);
+  load_vec<N>(p.inv_rbar + j0, irb);
+  load_vec<N>(p.rh_a + j0, ra);
+  load_vec<N>(p.rh_c + j0, rcc);
   C jt[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) jt[i] = (C)0;
   if constexpr (PLASMA) {
     const PlasmaTables<C>& pl = p.pl;
-    if (j0 + N > pl.band_lo && j0 < pl.band_hi) {
-      const C l0 = pl.left_tan[3 * e], l1 = pl.left_tan[3 * e + 1], l2 = pl.left_tan[3 * e + 2];
-      const C r0 = pl.right_tan[3 * e], r1 = pl.right_tan[3 * e + 1], r2 = pl.right_tan[3 * e + 2];
-#pragma unroll
-      for (int i = 0; i < N; ++i) {
-        const int32_t j = j0 + i;
-        if (j >= pl.band_lo && j < pl.band_hi)
-          jt[i] = (C)0.5 * (plasma_dot(pl, fl, j, l0, l1, l2) + plasma_dot(pl, fr, j, r0, r1, r2));
+    if (j0 >= pl.band_lo && j0 < pl.band_hi) {
+      C sl[N], sr[N], unused;
+      if constexpr (MAGNETIZED) {
+        C tl[4], tr[4];
+        load_vec<4>(pl.left_tan + 4 * e, tl);
+        load_vec<4>(pl.right_tan + 4 * e, tr);
+        plasma_project<C, N, false>(pl, fl, j0, tl, sl, &unused);
+        plasma_project<C, N, false>(pl, fr, j0, tr, sr, &unused);
+      } else {
+        const int32_t slot_l = pl.edge_slots[2 * e], slot_r = pl.edge_slots[2 * e + 1];
+        plasma_component<C, N, false>(pl, fl, 1 + slot_l, j0, sl, &unused);
+        plasma_component<C, N, false>(pl, fr, 1 + slot_r, j0, sr, &unused);
       }
+#pragma unroll
+      for (int i = 0; i < N; ++i) jt[i] = (C)0.5 * (sl[i] + sr[i]);
     }
   }
   C src[N];
@@ -594,15 +741,14 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, int32_t e, 
   }
 #pragma unroll
   for (int i = 0; i < N; ++i) {
-    const int32_t j = j0 + i;
-    const C grad = p.inv_rbar[j] * inv_phi * (hl[i] - hrr[i]);
-    const C rd = p.rh_a[j] * (ht[i + 1] - ht[i]) + p.rh_c[j] * ht[i + 1];
-    et[i] = ca[j] * et[i] + cb[j] * (grad - rd - jt[i] - src[i]);
+    const C grad = irb[i] * inv_phi * (hl[i] - hrr[i]);
+    const C rd = ra[i] * (ht[i + 1] - ht[i]) + rcc[i] * ht[i + 1];
+    et[i] = ca[i] * et[i] + cb[i] * (grad - rd - jt[i] - src[i]);
   }
   store_strip(erow, et, p.s_et);
 }
 
-template <typename T, typename C, bool PLASMA>
+template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
 __global__ void __launch_bounds__(kBlock) e_step_kernel(const EParams<T, C> p) {
   const int32_t L = logical_block(p.per_xcd);
   if (L >= p.n_blocks) return;
@@ -612,10 +758,10 @@ __global__ void __launch_bounds__(kBlock) e_step_kernel(const EParams<T, C> p) {
   int32_t row, strip;
   if (seg == kSegEVertex) {
     row_and_strip(p.g_vertex, b, &row, &strip);
-    if (row < p.g_vertex.n_rows) e_vertex_strip<T, C, PLASMA>(p, row, strip);
+    if (row < p.g_vertex.n_rows) e_vertex_strip<T, C, PLASMA, MAGNETIZED>(p, row, strip);
   } else {
     row_and_strip(p.g_edge, b, &row, &strip);
-    if (row < p.g_edge.n_rows) e_edge_strip<T, C, PLASMA>(p, row, strip);
+    if (row < p.g_edge.n_rows) e_edge_strip<T, C, PLASMA, MAGNETIZED>(p, row, strip);
   }
 }
 

@@ -386,6 +386,7 @@ class GeodesicFDTD:
                 ph.magnetized = int(pt["magnetized"])
                 ph.band_lo, ph.band_hi = lo, hi
                 ph.plasma_reconstruction = f64(pt["reconstruction"])
+                ph.face_tangents = f64(np.ascontiguousarray(pt["face_tangents"]))
                 ph.face_centers = f64(mesh.face_centers)
                 ph.left_tangents = f64(pt["left_tangents"])
                 ph.right_tangents = f64(pt["right_tangents"])
@@ -412,11 +413,14 @@ class GeodesicFDTD:
         self._ht_buf = torch.zeros((mesh.n_edges, pn), dtype=sdt, device=dev)
         self._et_buf = torch.zeros((mesh.n_edges, pm), dtype=sdt, device=dev)
         self._hr_buf = torch.zeros((mesh.n_faces, pm), dtype=sdt, device=dev)
-        self._si_memory = (torch.zeros((mesh.n_edges, si_terms), dtype=cdt, device=dev)
-                           if si_terms else None)
+        # term-major device layout; the reference's (edge, term) array is a view
+        self._si_buf = (torch.zeros((si_terms, mesh.n_edges), dtype=cdt, device=dev)
+                        if si_terms else None)
+        self._si_memory = self._si_buf.t() if si_terms else None
         n_species = ph.n_species
-        self._plasma_buf = (torch.zeros((n_species, 3, mesh.n_faces, layout.plasma_pitch),
-                                        dtype=cdt, device=dev) if n_species else None)
+        self._plasma_buf = (torch.zeros((n_species, layout.plasma_components, mesh.n_faces,
+                                         layout.plasma_pitch), dtype=cdt, device=dev)
+                            if n_species else None)
         self._surface_impedance_ade = (_SurfaceImpedanceState(self, si_terms) if si_terms
                                        else None)
         self._bind()
@@ -427,7 +431,7 @@ class GeodesicFDTD:
             ht=self._ht_buf.data_ptr(), ht_pitch=self._ht_buf.shape[1],
             et=self._et_buf.data_ptr(), et_pitch=self._et_buf.shape[1],
             hr=self._hr_buf.data_ptr(), hr_pitch=self._hr_buf.shape[1],
-            si_memory=self._si_memory.data_ptr() if self._si_memory is not None else None,
+            si_memory=self._si_buf.data_ptr() if self._si_buf is not None else None,
             plasma_current=self._plasma_buf.data_ptr() if self._plasma_buf is not None else None,
             plasma_pitch=self._layout.plasma_pitch)
         _native.check(self._lib.geo_plan_bind(self._plan, C.byref(st)), self._lib)
@@ -471,23 +475,43 @@ class GeodesicFDTD:
     ht = property(lambda self: self._field_view("ht"),
                   lambda self, v: self._set_field("ht", v))
 
+    def _plasma_bases(self):
+        """Device (normal, tangents, reconstruction) for state <-> Cartesian J."""
+        if getattr(self, "_plasma_basis_cache", None) is None:
+            pt = self._plasma_info
+            cdt = self.backend.compute_dtype
+            self._plasma_basis_cache = (self._tensor(self.mesh.face_centers, cdt),
+                                        self._tensor(pt["face_tangents"], cdt),
+                                        self._tensor(pt["reconstruction"], cdt))
+        return self._plasma_basis_cache
+
     @property
     def plasma_current_density(self) -> list:
-        """Per-species (face, radial, 3) current density (copies; zero off-band)."""
+        """Per-species (face, radial, 3) Cartesian current density (copies).
+
+        With B = 0 the device state holds the projections (J.n, J.t_c) the
+        scatter needs; J = (J.n) n + sum_c (J.t_c) R_c recovers the Cartesian
+        vector exactly (R = pinv(T).T spans the tangent plane)."""
+        torch = self.backend.torch
+        R = len(self.radial_midpoints_m)
         if self._plasma_buf is None:
             if self.plasma is None:
                 return []
-            torch = self.backend.torch
-            return [torch.zeros((self.mesh.n_faces, len(self.radial_midpoints_m), 3),
-                                dtype=self.backend.compute_dtype, device=self.backend.torch_device)
-                    for _ in self.plasma.species]
-        torch = self.backend.torch
+            return [torch.zeros((self.mesh.n_faces, R, 3), dtype=self.backend.compute_dtype,
+                                device=self.backend.torch_device) for _ in self.plasma.species]
         lo, hi = self._layout.band_lo, self._layout.band_hi
         out = []
         for s in range(self._plasma_buf.shape[0]):
-            full = torch.zeros((self.mesh.n_faces, len(self.radial_midpoints_m), 3),
-                               dtype=self._plasma_buf.dtype, device=self._plasma_buf.device)
-            full[:, lo:hi, :] = self._plasma_buf[s, :, :, : hi - lo].permute(1, 2, 0)
+            state = self._plasma_buf[s, :, :, : hi - lo]  # (comps, F, band)
+            if self._layout.plasma_components == 3:
+                band_j = state.permute(1, 2, 0)
+            else:
+                normal, _, recon = self._plasma_bases()
+                band_j = (state[0][..., None] * normal[:, None, :]
+                          + torch.einsum("cfb,fcd->fbd", state[1:], recon))
+            full = torch.zeros((self.mesh.n_faces, R, 3), dtype=self._plasma_buf.dtype,
+                               device=self._plasma_buf.device)
+            full[:, lo:hi, :] = band_j
             out.append(full)
         return out
 
@@ -504,7 +528,14 @@ class GeodesicFDTD:
         outside = torch_any_nonzero(data[:, :lo]) or torch_any_nonzero(data[:, hi:])
         if outside:
             raise ValueError("plasma current is nonzero outside the layers with plasma drive")
-        self._plasma_buf[index, :, :, : hi - lo] = data[:, lo:hi, :].permute(2, 0, 1)
+        band = data[:, lo:hi, :]
+        if self._layout.plasma_components == 3:
+            self._plasma_buf[index, :, :, : hi - lo] = band.permute(2, 0, 1)
+        else:
+            normal, tangents, _ = self._plasma_bases()
+            self._plasma_buf[index, 0, :, : hi - lo] = (band * normal[:, None, :]).sum(-1)
+            self._plasma_buf[index, 1:, :, : hi - lo] = self.backend.torch.einsum(
+                "fbd,fcd->cfb", band, tangents)
 
     # --------------------------------------------------------------- stepping
     def _stream(self):

@@ -106,6 +106,17 @@ def scenario(name: str, **overrides) -> dict:
     elif name == "c1":
         base.update(subdivision=7, radial_cells=64, dtype="float32", receivers="c1")
         base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3}
+    elif name == "c1_noplasma":
+        base.update(subdivision=7, radial_cells=64, dtype="float32", receivers="c1")
+    elif name == "c1_pec":
+        base.update(subdivision=7, radial_cells=64, dtype="float32", receivers="c1",
+                    surface_sigma=None)
+        base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3}
+    elif name in ("vacuum_s7_r64", "vacuum_s7_r80"):
+        # the reference's benchmark workload: vacuum, PEC, seeded fields, no source
+        base.update(subdivision=7, radial_cells=int(name[-2:]), dtype="float32",
+                    receivers="c1", vacuum=True, surface_sigma=None, source=None,
+                    seed_fields=True)
     else:
         raise KeyError(name)
     base.update(overrides)
