@@ -31,28 +31,32 @@ METRIC = "Mcell-updates/s + % HBM roofline (1 GPU); ensemble trace queries/s at 
 
 
 def algorithmic_bytes(sim) -> dict:
-    """Compulsory bytes per launch of each kernel.
+    """Compulsory bytes per launch of each kernel (the per-unit figures of the
+    step's byte model, split between the two half-step kernels).
 
-    Fields: every field value is read once by the half-step that uses it as a
-    source and read + written once by the half-step that updates it
-    (3 accesses per DOF and step).  Plasma current: read + written by the H
-    kernel (3 components per species and band cell).  Surface impedance:
-    memory read + written.  Tables: index / metric rows read once per entity.
+    Fields: every value is read once by the half-step that uses it as a source
+    and read + written once by the half-step that updates it (3 accesses per
+    DOF and step).  Plasma current: read + written once per step, 3 components
+    per species on the layers with plasma drive.  Surface impedance: memory
+    read + written.  Tables: index / metric rows read once per entity.
     """
     V, E, F = sim.mesh.n_vertices, sim.mesh.n_edges, sim.mesh.n_faces
     R = len(sim.radial_midpoints_m)
     sf = sim._er_buf.element_size()
     sc = 8 if sim.backend.dtype_name == "float64" else 4
-    lay = sim._layout
-    band = (lay.band_hi - lay.band_lo) if sim._plasma_buf is not None else 0
-    species = sim._plasma_buf.shape[0] if sim._plasma_buf is not None else 0
+    band = 0
+    species = 0
+    if sim._plasma_buf is not None:
+        lo, hi = sim._plasma_info["band"]
+        band = hi - lo
+        species = sim._plasma_buf.shape[0]
     si_terms = sim._si_memory.shape[1] if sim._si_memory is not None else 0
     h_fields = sf * (V * (R + 1) + E * R + 2 * E * (R + 1) + 2 * F * R)
     h_plasma = sc * 6 * F * band * species
     h_si = sc * 2 * si_terms * E
-    h_geom = 12 * E + 24 * F + (60 * F if species else 0)
+    h_geom = 12 * E + 24 * F + (48 * F if species else 0)
     e_fields = sf * (E * (R + 1) + F * R + 2 * V * (R + 1) + 2 * E * R)
-    e_geom = 48 * V + 12 * E + ((48 * V + 24 * E + 12 * F) if species else 0)
+    e_geom = 48 * V + 12 * E + ((48 * V + 8 * E) if species else 0)
     return {"h": h_fields + h_plasma + h_si + h_geom, "e": e_fields + e_geom,
             "h_fields": h_fields, "h_plasma": h_plasma, "h_si": h_si, "h_geom": h_geom,
             "e_fields": e_fields, "e_geom": e_geom}

@@ -179,6 +179,12 @@ typedef struct GeoDiagOut {
   double max_abs_er, max_abs_et, max_abs_hr, max_abs_ht;
   double energy_er, energy_et, energy_hr, energy_ht; /* 0.5*eps*E^2*w, 0.5*mu0*H^2*w */
   int64_t nonfinite;       /* count of non-finite field values */
+  /* Discrete div-B of the prism cells (face f, node k):
+   * D = Hr(f,k) Omega rbar_k^2 - Hr(f,k-1) Omega rbar_{k-1}^2
+   *     - sum_c s_fc theta_c r_k L_k Ht(e_c,k), with Hr(f,-1) = Hr(f,R) = 0;
+   * cells at k = 0 are skipped under a surface-impedance ground. */
+  double divb_max_abs;     /* max |D| */
+  double divb_flux_scale;  /* max over cells of the summed |flux| terms */
 } GeoDiagOut;
 
 typedef struct GeoPlan GeoPlan;

@@ -89,7 +89,8 @@ class GeoDiagOut(C.Structure):
                 ("max_abs_hr", C.c_double), ("max_abs_ht", C.c_double),
                 ("energy_er", C.c_double), ("energy_et", C.c_double),
                 ("energy_hr", C.c_double), ("energy_ht", C.c_double),
-                ("nonfinite", C.c_int64)]
+                ("nonfinite", C.c_int64),
+                ("divb_max_abs", C.c_double), ("divb_flux_scale", C.c_double)]
 
 
 EXPORTED_SYMBOLS = (

@@ -158,6 +158,51 @@ __global__ void diag_field_kernel(const T* __restrict__ x, int64_t rows, int32_t
   if (threadIdx.x == 0) partial[blockIdx.x] = {s_max[0], s_en[0], s_nf[0]};
 }
 
+// Discrete div-B per prism cell (f, k) in FP64; partial maxima of |D| and of
+// the flux scale per block.
+template <typename T, typename C>
+__global__ void diag_divb_kernel(const T* __restrict__ hr, const T* __restrict__ ht, int32_t pn,
+                                 int32_t pm, int32_t R, int64_t F, int32_t k_start,
+                                 C s_hr_inv, C s_ht_inv, const int32_t* __restrict__ face_edges,
+                                 const double* __restrict__ face_w,
+                                 const double* __restrict__ face_omega,
+                                 const double* __restrict__ rb2, const double* __restrict__ rl,
+                                 DiagAcc* __restrict__ partial) {
+  __shared__ double s_d[geo::kBlock];
+  __shared__ double s_s[geo::kBlock];
+  double dmax = 0.0, smax = 0.0;
+  const int64_t total = F * (R + 1);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = i / (R + 1);
+    const int32_t k = (int32_t)(i - f * (R + 1));
+    if (k < k_start) continue;
+    const double om = face_omega[f];
+    const double up = k < R ? (double)geo::load_one<T, C>(hr + f * pm + k, s_hr_inv) * om * rb2[k] : 0.0;
+    const double lo = k > 0 ? (double)geo::load_one<T, C>(hr + f * pm + k - 1, s_hr_inv) * om * rb2[k - 1] : 0.0;
+    double tang = 0.0, atang = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      const int32_t e = face_edges[3 * f + c];
+      const double flux = (double)geo::load_one<T, C>(ht + (int64_t)e * pn + k, s_ht_inv) * rl[k];
+      tang += face_w[3 * f + c] * flux;
+      atang += fabs(face_w[3 * f + c] * flux);
+    }
+    dmax = fmax(dmax, fabs(up - lo - tang));
+    smax = fmax(smax, fabs(up) + fabs(lo) + atang);
+  }
+  s_d[threadIdx.x] = dmax;
+  s_s[threadIdx.x] = smax;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      s_d[threadIdx.x] = fmax(s_d[threadIdx.x], s_d[threadIdx.x + s]);
+      s_s[threadIdx.x] = fmax(s_s[threadIdx.x], s_s[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = {s_d[0], s_s[0], 0};
+}
+
 __global__ void diag_reduce_kernel(const DiagAcc* __restrict__ partial, int32_t n,
                                    DiagAcc* __restrict__ out) {
   __shared__ double s_max[geo::kBlock];
@@ -183,6 +228,29 @@ __global__ void diag_reduce_kernel(const DiagAcc* __restrict__ partial, int32_t 
     __syncthreads();
   }
   if (threadIdx.x == 0) *out = {s_max[0], s_en[0], s_nf[0]};
+}
+
+// Final reduction for the div-B partials: both fields are maxima.
+__global__ void diag_reduce_max_kernel(const DiagAcc* __restrict__ partial, int32_t n,
+                                       DiagAcc* __restrict__ out) {
+  __shared__ double s_a[geo::kBlock];
+  __shared__ double s_b[geo::kBlock];
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    a = fmax(a, partial[i].max_abs);
+    b = fmax(b, partial[i].energy);
+  }
+  s_a[threadIdx.x] = a;
+  s_b[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      s_a[threadIdx.x] = fmax(s_a[threadIdx.x], s_a[threadIdx.x + s]);
+      s_b[threadIdx.x] = fmax(s_b[threadIdx.x], s_b[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = {s_a[0], s_b[0], 0};
 }
 
 template <typename T, typename C>
@@ -214,6 +282,7 @@ struct Plan final : PlanBase {
   double *w_hr_row = nullptr, *w_hr_col = nullptr, *w_ht_row = nullptr, *w_ht_col = nullptr;
   DiagAcc* diag_partial = nullptr;
   DiagAcc* diag_out = nullptr;
+  double *dv_face_w = nullptr, *dv_face_omega = nullptr, *dv_rb2 = nullptr, *dv_rl = nullptr;
   static constexpr int kDiagBlocks = 1024;
 
   ~Plan() override {
@@ -653,9 +722,23 @@ struct Plan final : PlanBase {
       if ((rc = arena.upload(ht_row, &w_ht_row)) != GEO_OK) return rc;
       if ((rc = arena.upload(ht_col, &w_ht_col)) != GEO_OK) return rc;
       void* p = nullptr;
-      if ((rc = arena.zeroed(sizeof(DiagAcc) * (kDiagBlocks * 4 + 4), &p)) != GEO_OK) return rc;
+      if ((rc = arena.zeroed(sizeof(DiagAcc) * (kDiagBlocks * 5 + 5), &p)) != GEO_OK) return rc;
       diag_partial = static_cast<DiagAcc*>(p);
-      diag_out = diag_partial + kDiagBlocks * 4;
+      diag_out = diag_partial + kDiagBlocks * 5;
+      // div-B tables: s_fc theta_c, Omega_f, rbar_j^2, r_k L_k
+      std::vector<double> fw(3 * F), fo(F), rb2(R), rl(R + 1);
+      for (int64_t f = 0; f < F; ++f) {
+        fo[f] = m->face_solid_angles[f];
+        for (int c = 0; c < 3; ++c)
+          fw[3 * f + c] = (double)m->face_edge_signs[3 * f + c] *
+                          m->primal_edge_angles[m->face_edges[3 * f + c]];
+      }
+      for (int64_t j = 0; j < R; ++j) rb2[j] = mid[j] * mid[j];
+      for (int64_t k = 0; k <= R; ++k) rl[k] = rad[k] * ctrl[k];
+      if ((rc = arena.upload(fw, &dv_face_w)) != GEO_OK) return rc;
+      if ((rc = arena.upload(fo, &dv_face_omega)) != GEO_OK) return rc;
+      if ((rc = arena.upload(rb2, &dv_rb2)) != GEO_OK) return rc;
+      if ((rc = arena.upload(rl, &dv_rl)) != GEO_OK) return rc;
     }
     layout.node_pitch = pn;
     layout.mid_pitch = pm;
@@ -832,7 +915,17 @@ struct Plan final : PlanBase {
     if ((rc = diag_one(hp.ht, E, (int32_t)(R + 1), (int32_t)pn, hp.s_ht_inv, w_ht_row,
                        w_ht_col, nullptr, 0, 0.5 * kMu0, 3, stream)) != GEO_OK)
       return rc;
-    DiagAcc host[4];
+    {
+      DiagAcc* part = diag_partial + 4 * kDiagBlocks;
+      hipLaunchKernelGGL((diag_divb_kernel<T, C>), dim3(kDiagBlocks), dim3(geo::kBlock), 0,
+                         stream, hp.hr, hp.ht, (int32_t)pn, (int32_t)pm, (int32_t)R, F,
+                         si_on ? 1 : 0, hp.s_hr_inv, hp.s_ht_inv, hp.face_edges, dv_face_w,
+                         dv_face_omega, dv_rb2, dv_rl, part);
+      hipLaunchKernelGGL(diag_reduce_max_kernel, dim3(1), dim3(geo::kBlock), 0, stream, part,
+                         kDiagBlocks, diag_out + 4);
+      GEO_HIP(hipGetLastError());
+    }
+    DiagAcc host[5];
     GEO_HIP(hipMemcpyAsync(host, diag_out, sizeof(host), hipMemcpyDeviceToHost, stream));
     GEO_HIP(hipStreamSynchronize(stream));
     out->max_abs_er = host[0].max_abs;
@@ -845,6 +938,8 @@ struct Plan final : PlanBase {
     out->energy_ht = host[3].energy;
     out->nonfinite = host[0].nonfinite + host[1].nonfinite + host[2].nonfinite +
                      host[3].nonfinite;
+    out->divb_max_abs = host[4].max_abs;
+    out->divb_flux_scale = host[4].energy;
     return GEO_OK;
   }
 

@@ -306,7 +306,10 @@ class GeodesicFDTD:
         torch = self.backend.torch
         if torch.is_tensor(values):
             return values.to(device=self.backend.torch_device, dtype=dtype)
-        return torch.as_tensor(np.asarray(values), dtype=dtype, device=self.backend.torch_device)
+        array = np.asarray(values)
+        if not array.flags.writeable:
+            array = array.copy()
+        return torch.as_tensor(array, dtype=dtype, device=self.backend.torch_device)
 
     def _create_plan(self) -> None:
         """Build the C-ABI descriptors, create the plan and bind the state."""

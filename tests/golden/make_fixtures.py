@@ -14,7 +14,11 @@ Outputs (data only, no reference code):
                             surface-impedance memory and plasma currents;
   * full_field.csv       -- artifacts/analytic-solutions/full-field.csv of the
                             reference (measured cavity frequencies and time
-                            steps), copied as data.
+                            steps), copied as data;
+  * checkpoint_<name>.npz -- a reference checkpoint (format version 4) after 60
+                            steps, and checkpoint_<name>_continued.npz the
+                            reference's fields after reloading it and running
+                            40 more steps.
 These pin the parts of the oracle that no closed form pins: integer mesh
 ordering, and the end-to-end conventions of the surface-impedance ADE, the
 plasma coupler and the source/receiver distributions.
@@ -164,7 +168,34 @@ def write_trajectory(name):
     print(name, "dt", sim.time_step_s, "max|er|", float(np.max(np.abs(out["er"]))), flush=True)
 
 
+def write_checkpoint_case(name, first=60, second=40):
+    """A reference checkpoint after `first` steps and the reference's own
+    continuation for `second` more steps after reloading it."""
+    sc = S.scenario(name)
+    sim = reference_sim(sc)
+    if sc["seed_fields"]:
+        S.apply_seed_fields(sim, lambda k, v: getattr(sim, k).__setitem__(slice(None), v))
+    sim.step(first)
+    path = HERE / f"checkpoint_{name}.npz"
+    sim.save_checkpoint(path)
+    resumed = GeodesicFDTD.load_checkpoint(path, dtype="float64")
+    resumed.step(second)
+    out = {k: np.asarray(getattr(resumed, k)) for k in ("er", "et", "hr", "ht")}
+    out["steps"] = np.int64(resumed.steps)
+    if resumed._surface_impedance_ade is not None:
+        out["si_memory"] = np.asarray(resumed._surface_impedance_ade.memory)
+    if resumed._plasma_coupler is not None:
+        for i, j in enumerate(resumed._plasma_coupler.ade.current_density):
+            out[f"plasma_current_{i}"] = np.asarray(j)
+    np.savez_compressed(HERE / f"checkpoint_{name}_continued.npz", **out)
+    print("checkpoint", name, resumed.steps, flush=True)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "checkpoints":
+        for name in ("c0_pulse", "plasma_random"):
+            write_checkpoint_case(name)
+        return
     if len(sys.argv) > 1:
         for name in sys.argv[1:]:
             write_trajectory(name)
@@ -186,6 +217,8 @@ def main():
         write_trajectory(name)
     shutil.copyfile(REFERENCE / "artifacts" / "analytic-solutions" / "full-field.csv",
                     HERE / "full_field.csv")
+    for name in ("c0_pulse", "plasma_random"):
+        write_checkpoint_case(name)
 
 
 if __name__ == "__main__":

@@ -212,3 +212,36 @@ def test_fp32_plasma_within_tolerance_of_reference(name):
     print(name, "fp32", {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     for k, err in errs.items():
         assert err <= FP32_BAR, k
+
+
+@pytest.mark.parametrize("dtype,bar", [("float64", 1e-13), ("float32", 1e-5)])
+def test_discrete_div_b_stays_at_roundoff_on_device(dtype, bar):
+    """Derived invariant: the H kernel conserves the prism-cell divergence of B;
+    from zero H and seeded E with a live source it stays at roundoff."""
+    sc = S.scenario("c0_pulse", surface_sigma=None)
+    sim = S.build_sim(sc, dtype=dtype, compile_step=True)
+    rng = np.random.default_rng(7)
+    sim.er = rng.standard_normal(tuple(sim.er.shape))
+    sim.et = rng.standard_normal(tuple(sim.et.shape))
+    sim.step(200)
+    d = sim.field_diagnostics()
+    print(dtype, "div-B", d["divb_max_abs"], "scale", d["divb_flux_scale"], flush=True)
+    assert d["divb_flux_scale"] > 0
+    assert d["divb_max_abs"] <= bar * d["divb_flux_scale"]
+    # and it matches the oracle's FP64 evaluation of the same formula on the device state
+    o = S.oracle_module()
+    D, scale = o.divergence_b(sim.mesh, sim.radii_m, sim.radial_midpoints_m,
+                              sim.radial_node_control_lengths_m, sim.to_numpy(sim.hr),
+                              sim.to_numpy(sim.ht))
+    assert d["divb_flux_scale"] == pytest.approx(float(np.max(scale)), rel=1e-12)
+    assert float(np.max(np.abs(D))) <= bar * float(np.max(scale))
+
+
+def test_div_b_detects_a_non_solenoidal_state():
+    """A random H is not divergence free: the diagnostic must see it."""
+    sc = S.scenario("pec_vacuum")
+    sim = S.build_sim(sc)
+    rng = np.random.default_rng(11)
+    sim.hr = rng.standard_normal(tuple(sim.hr.shape))
+    d = sim.field_diagnostics()
+    assert d["divb_max_abs"] > 1e-3 * d["divb_flux_scale"]
