@@ -107,6 +107,67 @@ def cpu_baseline(steps: int = 3) -> dict:
                       f"{os.cpu_count()} logical CPUs)"}
 
 
+def precision_sweep(args) -> dict:
+    """C3 precision sweep: the same grid and physics with FP64, FP32, FP16
+    (dynamic power-of-two scales) and BF16 field storage, FP32 update
+    arithmetic for the last three.  Errors against the FP64 run on the GPU:
+    receiver-trace relative L2, per-field relative L2 against the field
+    family's norm (Er, Et) / (Hr, Ht), and the discrete div-B residual
+    max |D| / flux scale.  Two initial states: zero fields driven by the
+    20 Hz source, and the reference benchmark's seeded random fields
+    (rng(20260814).standard_normal * 1e-6, source off) so that every cell is
+    active from the first step."""
+    import numpy as np
+    import torch
+
+    import scenarios as S
+
+    out = {}
+    for init in ("source", "seeded"):
+        sc = S.scenario(args.workload)
+        if init == "seeded":
+            sc.update(source=None, seed_fields=True)
+        ref_fields = ref_traces = None
+        res = {}
+        for dtype in ("float64", "float32", "float16", "bfloat16"):
+            t0 = time.perf_counter()
+            sim = S.build_sim(sc, dtype=dtype, compile_step=True, compile_chunk_size=args.chunk)
+            if sc["seed_fields"]:
+                S.apply_seed_fields(sim, lambda k, v: setattr(sim, k, v))
+            pts = S.receiver_points(sc)
+            v, l, w = S.receiver_tables(sim.mesh, sim.altitudes_m, pts)
+            setup = time.perf_counter() - t0
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            traces = sim.record_er_observations(v, l, w, args.steps)
+            torch.cuda.synchronize()
+            run = time.perf_counter() - t1
+            diag = sim.field_diagnostics()
+            fields = {k: getattr(sim, k).to(torch.float64) for k in ("er", "et", "hr", "ht")}
+            entry = {"ms_per_step": round(run * 1e3 / args.steps, 4),
+                     "divb_residual": diag["divb_max_abs"] / max(diag["divb_flux_scale"], 1e-300),
+                     "finite": bool(diag["nonfinite"] == 0 and diag["storage_overflow"] == 0),
+                     "trace_norm": float(np.linalg.norm(traces))}
+            if dtype == "float64":
+                ref_fields, ref_traces = fields, traces
+            else:
+                entry["trace_rel_l2"] = S.rel_l2(traces, ref_traces)
+                for fam in (("er", "et"), ("hr", "ht")):
+                    norm = torch.sqrt(sum(torch.sum(ref_fields[k] ** 2) for k in fam))
+                    for k in fam:
+                        entry[f"{k}_rel_l2"] = float(torch.linalg.norm(fields[k] - ref_fields[k])
+                                                     / norm)
+                if dtype in ("float16", "bfloat16"):
+                    entry["final_scales"] = sim.storage_scales
+            res[dtype] = entry
+            print(f"precision {init} {dtype}: setup {setup:.1f} s, run {run:.1f} s, {entry}",
+                  file=sys.stderr, flush=True)
+            del sim, fields
+            torch.cuda.empty_cache()
+        out[init] = res
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,11 +177,15 @@ def main() -> int:
     ap.add_argument("--kernel-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="c1",
-                    help="c1 (default), c1_noplasma, c1_pec, vacuum_s7_r64, vacuum_s7_r80")
+                    help="c1 (default), c1_noplasma, c1_pec, c3, vacuum_s7_r64, vacuum_s7_r80")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="also time a 1 GiB device copy (read + write GB/s)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run 4 torch copies of 512 MiB (counter calibration)")
+    ap.add_argument("--precision-sweep", action="store_true",
+                    help="run the FP64/FP32/FP16/BF16 storage sweep on --workload (c3: s8 x 96, "
+                         "refined 60-90 km band) for --steps steps instead of the throughput "
+                         "bench")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the multi-GPU halo path is out of scope for this "
@@ -132,6 +197,12 @@ def main() -> int:
     import scenarios as S
 
     torch.cuda.set_device(0)
+    if args.precision_sweep:
+        sweep = precision_sweep(args)
+        print(json.dumps({"metric": "storage precision sweep: error vs FP64 GPU run",
+                          "workload": args.workload, "steps": args.steps,
+                          "graph_chunk": args.chunk, "results": sweep}))
+        return 0
     sc = S.scenario(args.workload)
     t_setup = time.perf_counter()
     sim = S.build_sim(sc, compile_step=True, compile_chunk_size=args.chunk)

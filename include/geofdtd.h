@@ -99,7 +99,8 @@ typedef struct GeoRadialDesc {
 /* Time step, loss coefficients, boundary, sources and plasma. */
 typedef struct GeoPhysicsDesc {
   int32_t storage;           /* GEO_STORAGE_* */
-  int32_t reserved0;
+  int32_t scaling;           /* narrow storage: 0 static scales, 1 dynamic power-of-two
+                                rescaling before every chunk of steps */
   double time_step_s;
   /* Exponential / trapezoidal loss coefficients.  coef_rows = 1 means one
    * horizontally uniform radial profile (solver.py:737-748). */
@@ -185,6 +186,7 @@ typedef struct GeoDiagOut {
    * cells at k = 0 are skipped under a surface-impedance ground. */
   double divb_max_abs;     /* max |D| */
   double divb_flux_scale;  /* max over cells of the summed |flux| terms */
+  int64_t storage_overflow; /* 1 once a rescale point saw a non-finite stored value */
 } GeoDiagOut;
 
 typedef struct GeoPlan GeoPlan;
@@ -211,6 +213,13 @@ int geo_diagnostics(GeoPlan* plan, GeoDiagOut* out, void* stream);
 /* Average kernel durations (ms) of n_steps individually launched and
  * event-timed steps: out_ms[0] = H kernel, out_ms[1] = E kernel. */
 int geo_time_kernels(GeoPlan* plan, int64_t n_steps, void* stream, double* out_ms);
+/* Current storage scales (stored = value * scale) of er, et, hr, ht; 1 for
+ * FP32/FP64 storage.  Synchronises the stream.  No reference counterpart: the
+ * reference stores fields in FP32/FP64 only. */
+int geo_get_storage_scales(GeoPlan* plan, double* out4, void* stream);
+/* Replace the storage scales (powers of two in [2^-60, 2^60]) after the
+ * caller has written field values at those scales.  Synchronises the stream. */
+int geo_set_storage_scales(GeoPlan* plan, const double* in4, void* stream);
 void geo_plan_destroy(GeoPlan* plan);
 /* Bandwidth probe (no counterpart in the reference): average time (ms) of a
  * 16-byte-strip kernel over `bytes` -- mode 0 copy src->dst, 1 read src,

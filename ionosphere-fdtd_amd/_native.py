@@ -45,7 +45,7 @@ class GeoRadialDesc(C.Structure):
 
 class GeoPhysicsDesc(C.Structure):
     _fields_ = [
-        ("storage", C.c_int32), ("reserved0", C.c_int32), ("time_step_s", C.c_double),
+        ("storage", C.c_int32), ("scaling", C.c_int32), ("time_step_s", C.c_double),
         ("ca_er", _p_f64), ("cb_er", _p_f64), ("er_coef_rows", C.c_int64),
         ("ca_et", _p_f64), ("cb_et", _p_f64), ("et_coef_rows", C.c_int64),
         ("eps_er", _p_f64), ("eps_et", _p_f64),
@@ -90,13 +90,15 @@ class GeoDiagOut(C.Structure):
                 ("energy_er", C.c_double), ("energy_et", C.c_double),
                 ("energy_hr", C.c_double), ("energy_ht", C.c_double),
                 ("nonfinite", C.c_int64),
-                ("divb_max_abs", C.c_double), ("divb_flux_scale", C.c_double)]
+                ("divb_max_abs", C.c_double), ("divb_flux_scale", C.c_double),
+                ("storage_overflow", C.c_int64)]
 
 
 EXPORTED_SYMBOLS = (
     "geo_plan_create", "geo_plan_layout", "geo_plan_bind", "geo_step", "geo_set_receivers",
     "geo_sample_er", "geo_diagnostics", "geo_time_kernels", "geo_plan_destroy",
-    "geo_last_error", "geo_version", "geo_stream_probe",
+    "geo_last_error", "geo_version", "geo_stream_probe", "geo_get_storage_scales",
+    "geo_set_storage_scales",
 )
 
 _lib = None
@@ -123,6 +125,8 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.geo_diagnostics.argtypes = [vp, C.POINTER(GeoDiagOut), vp]
     lib.geo_time_kernels.argtypes = [vp, C.c_int64, vp, _p_f64]
     lib.geo_stream_probe.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, _p_f64]
+    lib.geo_get_storage_scales.argtypes = [vp, _p_f64, vp]
+    lib.geo_set_storage_scales.argtypes = [vp, _p_f64, vp]
     lib.geo_plan_destroy.argtypes = [vp]
     lib.geo_plan_destroy.restype = None
     lib.geo_last_error.restype = C.c_char_p

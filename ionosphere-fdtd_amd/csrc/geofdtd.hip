@@ -8,7 +8,9 @@
 #include "geofdtd_kernels.h"
 
 #include <algorithm>
+#include <cfloat>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -33,6 +35,16 @@ int fail(int code, const std::string& msg) {
   } while (0)
 
 int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+// A storage scale: a power of two in [2^-60, 2^60], so multiplying by it or
+// by a ratio of two of them is exact in FP32 and in the narrow types' normal
+// range.
+bool valid_scale(double s) {
+  if (!(s > 0.0) || !std::isfinite(s)) return false;
+  int e = 0;
+  const double m = std::frexp(s, &e);
+  return m == 0.5 && e - 1 >= -60 && e - 1 <= 60;
+}
 
 // Owns device allocations made at plan creation.
 struct DeviceArena {
@@ -108,6 +120,8 @@ struct PlanBase {
   virtual int sample(void* row, hipStream_t stream) = 0;
   virtual int diagnostics(GeoDiagOut* out, hipStream_t stream) = 0;
   virtual int time_kernels(int64_t n, hipStream_t stream, double* out_ms) = 0;
+  virtual int get_scales(double* out, hipStream_t stream) = 0;
+  virtual int set_scales(const double* in, hipStream_t stream) = 0;
   GeoLayout layout{};
 };
 
@@ -120,7 +134,8 @@ struct DiagAcc {
 
 template <typename T, typename C>
 __global__ void diag_field_kernel(const T* __restrict__ x, int64_t rows, int32_t cols,
-                                  int32_t pitch, C inv_scale, const double* __restrict__ row_w,
+                                  int32_t pitch, const geo::Dynamic* __restrict__ dyn,
+                                  int32_t field, const double* __restrict__ row_w,
                                   const double* __restrict__ col_w,
                                   const double* __restrict__ eps, int64_t eps_stride,
                                   double coef, DiagAcc* __restrict__ partial) {
@@ -129,6 +144,7 @@ __global__ void diag_field_kernel(const T* __restrict__ x, int64_t rows, int32_t
   __shared__ long long s_nf[geo::kBlock];
   double mx = 0.0, en = 0.0;
   long long nf = 0;
+  const C inv_scale = geo::is_narrow<T>() ? (C)dyn->inv_scale[field] : (C)1;
   const int64_t total = rows * cols;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -163,7 +179,8 @@ __global__ void diag_field_kernel(const T* __restrict__ x, int64_t rows, int32_t
 template <typename T, typename C>
 __global__ void diag_divb_kernel(const T* __restrict__ hr, const T* __restrict__ ht, int32_t pn,
                                  int32_t pm, int32_t R, int64_t F, int32_t k_start,
-                                 C s_hr_inv, C s_ht_inv, const int32_t* __restrict__ face_edges,
+                                 const geo::Dynamic* __restrict__ dyn,
+                                 const int32_t* __restrict__ face_edges,
                                  const double* __restrict__ face_w,
                                  const double* __restrict__ face_omega,
                                  const double* __restrict__ rb2, const double* __restrict__ rl,
@@ -171,6 +188,8 @@ __global__ void diag_divb_kernel(const T* __restrict__ hr, const T* __restrict__
   __shared__ double s_d[geo::kBlock];
   __shared__ double s_s[geo::kBlock];
   double dmax = 0.0, smax = 0.0;
+  const geo::Scales<C> sc = geo::load_scales<T, C>(dyn);
+  const C s_hr_inv = sc.hr_inv, s_ht_inv = sc.ht_inv;
   const int64_t total = F * (R + 1);
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -253,6 +272,151 @@ __global__ void diag_reduce_max_kernel(const DiagAcc* __restrict__ partial, int3
   if (threadIdx.x == 0) *out = {s_a[0], s_b[0], 0};
 }
 
+// ---------------------------------------------------------------------------
+// Dynamic power-of-two rescaling of narrow (FP16/BF16) field storage.
+//
+// Before every block of steps (one graph chunk), three small launches:
+//   scale_max_kernel     per-block maxima of |stored value| of each field;
+//   scale_decide_kernel  one block: physical maxima, the source drive over
+//                        the coming steps, and a new power-of-two scale for
+//                        every field whose stored bound would leave
+//                        [2^7, 2^12] (re-placed at 2^10: 64x headroom below
+//                        the FP16 limit 65504; precision does not depend on
+//                        the placement while values stay above 2^-14);
+//   scale_apply_kernel   multiplies the stored values of the rescaled fields
+//                        by the exact ratio of the two scales.
+// All state lives in the device block, so captured graphs stay valid.
+// ---------------------------------------------------------------------------
+constexpr int kScaleBlocks = 512;
+constexpr int kRescaleInterval = 32;  // steps between rescale points without graphs
+constexpr int kMaxScaleExp = 60;
+static_assert(offsetof(geo::Dynamic, inv_scale) == offsetof(geo::Dynamic, scale) + 4 * sizeof(float),
+              "set_scales writes scale[] and inv_scale[] with one copy");
+
+template <typename T>
+__global__ void __launch_bounds__(geo::kBlock) scale_max_kernel(
+    const T* __restrict__ er, const T* __restrict__ et, const T* __restrict__ hr,
+    const T* __restrict__ ht, int64_t n_er, int64_t n_et, int64_t n_hr, int64_t n_ht,
+    float* __restrict__ partial) {
+  __shared__ float red[geo::kBlock];
+  const T* base[4] = {er, et, hr, ht};
+  const int64_t strips[4] = {n_er, n_et, n_hr, n_ht};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int f = 0; f < 4; ++f) {
+    const geo::Strip<T>* src = reinterpret_cast<const geo::Strip<T>*>(base[f]);
+    float m = 0.f;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < strips[f];
+         i += stride) {
+      const geo::Strip<T> v = src[i];
+#pragma unroll
+      for (int l = 0; l < geo::Strip<T>::N; ++l) {
+        const float a = fabsf(geo::to_c<float>(v.v[l]));
+        m = (a <= FLT_MAX) ? fmaxf(m, a) : INFINITY;  // NaN and inf count as inf
+      }
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[4 * blockIdx.x + f] = red[0];
+    __syncthreads();
+  }
+}
+
+// gain_er / gain_et bound the change of Er / Et per step and ampere of source
+// current (sum over support points of |Cb * factor|).  The bound of a field
+// is that of its family (Er with Et, Hr with Ht) and of the other family
+// across the impedance of free space (377 ohm ~ 2^8): a field at roundoff
+// level (Hr under a vertical source) or one about to be driven (H fields
+// before the first wave, E fields seeded far below eta0 * H) is scaled for
+// the values the coupled update can bring within the next chunk.
+__global__ void __launch_bounds__(geo::kBlock) scale_decide_kernel(
+    geo::Dynamic* __restrict__ dyn, const float* __restrict__ partial, int32_t n_partial,
+    int32_t interval, float gain_er, float gain_et) {
+  __shared__ float red[5][geo::kBlock];
+  const int t = threadIdx.x;
+  float m[4] = {0.f, 0.f, 0.f, 0.f};
+  for (int b = t; b < n_partial; b += blockDim.x)
+    for (int f = 0; f < 4; ++f) m[f] = fmaxf(m[f], partial[4 * b + f]);
+  // the step that starts with the device counter at c injects currents[c]
+  float drive = 0.f;
+  const float* cur = static_cast<const float*>(dyn->currents);
+  if (cur != nullptr) {
+    const int64_t c0 = dyn->counter;
+    const int64_t c1 = c0 + interval < dyn->n_currents ? c0 + interval : dyn->n_currents;
+    for (int64_t i = c0 + t; i < c1; i += blockDim.x) drive += fabsf(cur[i]);
+  }
+  for (int f = 0; f < 4; ++f) red[f][t] = m[f];
+  red[4][t] = drive;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      for (int f = 0; f < 4; ++f) red[f][t] = fmaxf(red[f][t], red[f][t + s]);
+      red[4][t] += red[4][t + s];
+    }
+    __syncthreads();
+  }
+  if (t != 0) return;
+  constexpr float kHi = 4096.f, kLo = 128.f, kTarget = 1024.f, kEta = 256.f;
+  float phys[4];
+  bool bad = false;
+  for (int f = 0; f < 4; ++f) {
+    if (!(red[f][0] <= FLT_MAX)) bad = true;
+    phys[f] = red[f][0] * dyn->inv_scale[f];
+  }
+  if (bad) {
+    dyn->overflow = 1;
+    for (int f = 0; f < 4; ++f) dyn->factor[f] = 1.f;
+    return;
+  }
+  const float d = red[4][0];
+  const float b[4] = {phys[0] + gain_er * d, phys[1] + gain_et * d, phys[2], phys[3]};
+  const float eb = fmaxf(b[0], b[1]), hb = fmaxf(b[2], b[3]);
+  const float be = fmaxf(eb, kEta * hb), bh = fmaxf(hb, eb / kEta);
+  const float bounds[4] = {be, be, bh, bh};
+  for (int f = 0; f < 4; ++f) {
+    const float bound = bounds[f];
+    float fac = 1.f;
+    if (bound > 0.f && bound <= FLT_MAX) {
+      const float stored = bound * dyn->scale[f];
+      if (stored > kHi || stored < kLo) {
+        int k = (int)floorf(log2f(kTarget / bound));
+        k = k > kMaxScaleExp ? kMaxScaleExp : (k < -kMaxScaleExp ? -kMaxScaleExp : k);
+        const float ns = ldexpf(1.f, k);
+        if (phys[f] > 0.f) fac = ns * dyn->inv_scale[f];  // ratio of powers of two: exact
+        dyn->scale[f] = ns;
+        dyn->inv_scale[f] = ldexpf(1.f, -k);
+      }
+    }
+    dyn->factor[f] = fac;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(geo::kBlock) scale_apply_kernel(
+    T* __restrict__ er, T* __restrict__ et, T* __restrict__ hr, T* __restrict__ ht,
+    int64_t n_er, int64_t n_et, int64_t n_hr, int64_t n_ht,
+    const geo::Dynamic* __restrict__ dyn) {
+  T* base[4] = {er, et, hr, ht};
+  const int64_t strips[4] = {n_er, n_et, n_hr, n_ht};
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int f = 0; f < 4; ++f) {
+    const float fac = dyn->factor[f];
+    if (fac == 1.f) continue;
+    geo::Strip<T>* dst = reinterpret_cast<geo::Strip<T>*>(base[f]);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < strips[f];
+         i += stride) {
+      geo::Strip<T> v = dst[i];
+#pragma unroll
+      for (int l = 0; l < geo::Strip<T>::N; ++l)
+        v.v[l] = geo::from_c<T>(geo::to_c<float>(v.v[l]) * fac);
+      dst[i] = v;
+    }
+  }
+}
+
 template <typename T, typename C>
 struct Plan final : PlanBase {
   static constexpr int N = geo::Strip<T>::N;
@@ -265,6 +429,10 @@ struct Plan final : PlanBase {
   int32_t h_grid = 0, e_grid = 0;
   geo::Dynamic* dyn = nullptr;
   bool bound = false;
+  // narrow storage with dynamic power-of-two scales
+  bool dyn_scaling = false;
+  float src_gain_er = 0.f, src_gain_et = 0.f;
+  float* scale_partial = nullptr;
   // receivers (owned separately so repeated record calls do not grow the arena)
   int32_t n_rx = 0;
   void* rx_mem = nullptr;
@@ -425,6 +593,12 @@ struct Plan final : PlanBase {
     const bool narrow = sizeof(T) == 2;
     const double s_er = narrow ? ph->scale_er : 1.0, s_et = narrow ? ph->scale_et : 1.0;
     const double s_hr = narrow ? ph->scale_hr : 1.0, s_ht = narrow ? ph->scale_ht : 1.0;
+    for (const double sv : {s_er, s_et, s_hr, s_ht})
+      if (!valid_scale(sv))
+        return fail(GEO_ERR_INVALID, "storage scales must be powers of two in [2^-60, 2^60]");
+    if (ph->scaling != 0 && ph->scaling != 1)
+      return fail(GEO_ERR_INVALID, "storage scaling must be 0 (static) or 1 (dynamic)");
+    dyn_scaling = narrow && ph->scaling == 1;
     // ---- surface impedance ----
     si_terms = ph->si_terms;
     si_on = si_terms > 0;
@@ -554,6 +728,21 @@ struct Plan final : PlanBase {
     void* dyn_mem = nullptr;
     if ((rc = arena.zeroed(sizeof(geo::Dynamic), &dyn_mem)) != GEO_OK) return rc;
     dyn = static_cast<geo::Dynamic*>(dyn_mem);
+    {
+      geo::Dynamic init{};
+      const double s0[4] = {s_er, s_et, s_hr, s_ht};
+      for (int f = 0; f < 4; ++f) {
+        init.scale[f] = (float)s0[f];
+        init.inv_scale[f] = (float)(1.0 / s0[f]);
+        init.factor[f] = 1.f;
+      }
+      GEO_HIP(hipMemcpy(dyn, &init, sizeof(init), hipMemcpyHostToDevice));
+    }
+    if (dyn_scaling) {
+      void* sp = nullptr;
+      if ((rc = arena.zeroed(sizeof(float) * 4 * kScaleBlocks, &sp)) != GEO_OK) return rc;
+      scale_partial = static_cast<float*>(sp);
+    }
 
     geo::PlasmaTables<C> pl{};
     if (plasma) {
@@ -582,12 +771,6 @@ struct Plan final : PlanBase {
     hp.pm = (int32_t)pm;
     hp.R = (int32_t)R;
     hp.si_on = si_on ? 1 : 0;
-    hp.s_er_inv = (C)(1.0 / s_er);
-    hp.s_et_inv = (C)(1.0 / s_et);
-    hp.s_ht = (C)s_ht;
-    hp.s_ht_inv = (C)(1.0 / s_ht);
-    hp.s_hr = (C)s_hr;
-    hp.s_hr_inv = (C)(1.0 / s_hr);
     hp.edges = d_edges;
     hp.edge_grad = d_eg;
     hp.face_edges = d_fe;
@@ -638,12 +821,6 @@ struct Plan final : PlanBase {
     ep.R = (int32_t)R;
     ep.ca_er_stride = er_rows == 1 ? 0 : (int32_t)pn;
     ep.ca_et_stride = et_rows == 1 ? 0 : (int32_t)pm;
-    ep.s_er = (C)s_er;
-    ep.s_er_inv = (C)(1.0 / s_er);
-    ep.s_et = (C)s_et;
-    ep.s_et_inv = (C)(1.0 / s_et);
-    ep.s_ht_inv = (C)(1.0 / s_ht);
-    ep.s_hr_inv = (C)(1.0 / s_hr);
     ep.vertex_order = d_vo;
     ep.vertex_edges = d_ve;
     ep.vertex_w = d_vw;
@@ -666,8 +843,10 @@ struct Plan final : PlanBase {
       ep.src_row[q] = (int32_t)v;
       ep.src_layer[q] = (int32_t)l;
       // solver.py:936-945: w * l / (A_v r_k^2 L_k)
-      ep.src_factor[q] = (C)(ph->source_weights[q] * ph->source_length_m /
-                             m->dual_cell_solid_angles[v] / (rad[l] * rad[l]) / ctrl[l]);
+      const double factor = ph->source_weights[q] * ph->source_length_m /
+                            m->dual_cell_solid_angles[v] / (rad[l] * rad[l]) / ctrl[l];
+      ep.src_factor[q] = (C)factor;
+      src_gain_er += (float)std::fabs(factor * ph->cb_er[(er_rows == 1 ? 0 : v) * (R + 1) + l]);
     }
     ep.n_tsrc = ph->n_tsource;
     for (int q = 0; q < ph->n_tsource; ++q) {
@@ -677,8 +856,9 @@ struct Plan final : PlanBase {
       ep.tsrc_row[q] = (int32_t)e;
       ep.tsrc_layer[q] = (int32_t)l;
       // solver.py:973-981: w / (phi_e rbar_j dr_j)
-      ep.tsrc_factor[q] =
-          (C)(ph->tsource_weights[q] / m->dual_edge_angles[e] / mid[l] / dr[l]);
+      const double factor = ph->tsource_weights[q] / m->dual_edge_angles[e] / mid[l] / dr[l];
+      ep.tsrc_factor[q] = (C)factor;
+      src_gain_et += (float)std::fabs(factor * ph->cb_et[(et_rows == 1 ? 0 : e) * R + l]);
     }
     ep.dyn = dyn;
     ep.g_vertex = make_geom(V, pn, N);
@@ -833,10 +1013,25 @@ struct Plan final : PlanBase {
     }
   }
 
+  void launch_rescale(hipStream_t s, int32_t interval) {
+    if constexpr (sizeof(T) == 2) {
+      if (!dyn_scaling) return;
+      const int64_t n_er = V * pn / N, n_et = E * pm / N, n_hr = F * pm / N, n_ht = E * pn / N;
+      hipLaunchKernelGGL((scale_max_kernel<T>), dim3(kScaleBlocks), dim3(geo::kBlock), 0, s,
+                         hp.er, hp.et, hp.hr, hp.ht, n_er, n_et, n_hr, n_ht, scale_partial);
+      hipLaunchKernelGGL(scale_decide_kernel, dim3(1), dim3(geo::kBlock), 0, s, dyn,
+                         scale_partial, kScaleBlocks, interval, src_gain_er, src_gain_et);
+      hipLaunchKernelGGL((scale_apply_kernel<T>), dim3(kScaleBlocks), dim3(geo::kBlock), 0, s,
+                         ep.er, ep.et, hp.hr, hp.ht, n_er, n_et, n_hr, n_ht,
+                         (const geo::Dynamic*)dyn);
+    }
+  }
+
   int ensure_graph(int32_t chunk) {
     if (graph_exec && graph_chunk == chunk) return GEO_OK;
     drop_graph();
     GEO_HIP(hipStreamBeginCapture(capture_stream, hipStreamCaptureModeThreadLocal));
+    launch_rescale(capture_stream, chunk);
     for (int32_t i = 0; i < chunk; ++i) {
       launch_h(capture_stream);
       launch_e(capture_stream);
@@ -856,8 +1051,11 @@ struct Plan final : PlanBase {
     if (trace != nullptr && n_rx <= 0)
       return fail(GEO_ERR_INVALID, "trace recording requires receivers");
     hipLaunchKernelGGL(geo::set_dynamic_kernel, dim3(1), dim3(1), 0, stream, dyn, (int64_t)0,
-                       trace, trace ? n : (int64_t)0, currents);
+                       trace, trace ? n : (int64_t)0, currents, n);
     GEO_HIP(hipGetLastError());
+    // rescale points (narrow storage): every `interval` steps from the start
+    // of the call, identical with and without graphs for the same chunk
+    const int32_t interval = chunk > 1 ? chunk : kRescaleInterval;
     int64_t done = 0;
     if (chunk > 1 && n >= chunk) {
       int rc = ensure_graph(chunk);
@@ -865,6 +1063,7 @@ struct Plan final : PlanBase {
       for (; done + chunk <= n; done += chunk) GEO_HIP(hipGraphLaunch(graph_exec, stream));
     }
     for (; done < n; ++done) {
+      if (done % interval == 0) launch_rescale(stream, interval);
       launch_h(stream);
       launch_e(stream);
     }
@@ -881,18 +1080,20 @@ struct Plan final : PlanBase {
     if (n_rx <= 0) return fail(GEO_ERR_INVALID, "no receivers are set");
     const int threads = 64;
     hipLaunchKernelGGL((geo::sample_kernel<T, C>), dim3((n_rx + threads - 1) / threads),
-                       dim3(threads), 0, stream, hp.er, hp.pn, hp.s_er_inv, hp.rx_vertex,
+                       dim3(threads), 0, stream, hp.er, hp.pn, (const geo::Dynamic*)dyn,
+                       hp.rx_vertex,
                        hp.rx_layer, hp.rx_w, n_rx, static_cast<C*>(row));
     GEO_HIP(hipGetLastError());
     return GEO_OK;
   }
 
-  int diag_one(const T* x, int64_t rows, int32_t cols, int32_t pitch, C inv_scale,
+  int diag_one(const T* x, int64_t rows, int32_t cols, int32_t pitch, int32_t field,
                const double* row_w, const double* col_w, const double* eps, int64_t eps_stride,
                double coef, int slot, hipStream_t stream) {
     DiagAcc* part = diag_partial + slot * kDiagBlocks;
     hipLaunchKernelGGL((diag_field_kernel<T, C>), dim3(kDiagBlocks), dim3(geo::kBlock), 0,
-                       stream, x, rows, cols, pitch, inv_scale, row_w, col_w, eps, eps_stride,
+                       stream, x, rows, cols, pitch, (const geo::Dynamic*)dyn, field, row_w,
+                       col_w, eps, eps_stride,
                        coef, part);
     hipLaunchKernelGGL(diag_reduce_kernel, dim3(1), dim3(geo::kBlock), 0, stream, part,
                        kDiagBlocks, diag_out + slot);
@@ -903,31 +1104,35 @@ struct Plan final : PlanBase {
   int diagnostics(GeoDiagOut* out, hipStream_t stream) override {
     if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
     int rc;
-    if ((rc = diag_one(hp.er, V, (int32_t)(R + 1), (int32_t)pn, hp.s_er_inv, w_er_row,
+    if ((rc = diag_one(hp.er, V, (int32_t)(R + 1), (int32_t)pn, geo::kFieldEr, w_er_row,
                        w_er_col, eps_er_dev, eps_er_stride, 0.5, 0, stream)) != GEO_OK)
       return rc;
-    if ((rc = diag_one(hp.et, E, (int32_t)R, (int32_t)pm, hp.s_et_inv, w_et_row, w_et_col,
+    if ((rc = diag_one(hp.et, E, (int32_t)R, (int32_t)pm, geo::kFieldEt, w_et_row, w_et_col,
                        eps_et_dev, eps_et_stride, 0.5, 1, stream)) != GEO_OK)
       return rc;
-    if ((rc = diag_one(hp.hr, F, (int32_t)R, (int32_t)pm, hp.s_hr_inv, w_hr_row, w_hr_col,
+    if ((rc = diag_one(hp.hr, F, (int32_t)R, (int32_t)pm, geo::kFieldHr, w_hr_row, w_hr_col,
                        nullptr, 0, 0.5 * kMu0, 2, stream)) != GEO_OK)
       return rc;
-    if ((rc = diag_one(hp.ht, E, (int32_t)(R + 1), (int32_t)pn, hp.s_ht_inv, w_ht_row,
+    if ((rc = diag_one(hp.ht, E, (int32_t)(R + 1), (int32_t)pn, geo::kFieldHt, w_ht_row,
                        w_ht_col, nullptr, 0, 0.5 * kMu0, 3, stream)) != GEO_OK)
       return rc;
     {
       DiagAcc* part = diag_partial + 4 * kDiagBlocks;
       hipLaunchKernelGGL((diag_divb_kernel<T, C>), dim3(kDiagBlocks), dim3(geo::kBlock), 0,
                          stream, hp.hr, hp.ht, (int32_t)pn, (int32_t)pm, (int32_t)R, F,
-                         si_on ? 1 : 0, hp.s_hr_inv, hp.s_ht_inv, hp.face_edges, dv_face_w,
+                         si_on ? 1 : 0, (const geo::Dynamic*)dyn, hp.face_edges, dv_face_w,
                          dv_face_omega, dv_rb2, dv_rl, part);
       hipLaunchKernelGGL(diag_reduce_max_kernel, dim3(1), dim3(geo::kBlock), 0, stream, part,
                          kDiagBlocks, diag_out + 4);
       GEO_HIP(hipGetLastError());
     }
     DiagAcc host[5];
+    int32_t overflow = 0;
     GEO_HIP(hipMemcpyAsync(host, diag_out, sizeof(host), hipMemcpyDeviceToHost, stream));
+    GEO_HIP(hipMemcpyAsync(&overflow, &dyn->overflow, sizeof(overflow), hipMemcpyDeviceToHost,
+                           stream));
     GEO_HIP(hipStreamSynchronize(stream));
+    out->storage_overflow = overflow;
     out->max_abs_er = host[0].max_abs;
     out->max_abs_et = host[1].max_abs;
     out->max_abs_hr = host[2].max_abs;
@@ -947,7 +1152,8 @@ struct Plan final : PlanBase {
     if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
     if (n <= 0) return fail(GEO_ERR_INVALID, "step count must be positive");
     hipLaunchKernelGGL(geo::set_dynamic_kernel, dim3(1), dim3(1), 0, stream, dyn, (int64_t)0,
-                       (void*)nullptr, (int64_t)0, (const void*)nullptr);
+                       (void*)nullptr, (int64_t)0, (const void*)nullptr, (int64_t)0);
+    launch_rescale(stream, (int32_t)std::min<int64_t>(n, 1 << 30));
     std::vector<hipEvent_t> ev(3 * n);
     for (auto& e : ev) GEO_HIP(hipEventCreate(&e));
     for (int64_t i = 0; i < n; ++i) {
@@ -970,6 +1176,30 @@ struct Plan final : PlanBase {
     for (auto& x : ev) (void)hipEventDestroy(x);
     out_ms[0] = h / n;
     out_ms[1] = e / n;
+    return GEO_OK;
+  }
+
+  int get_scales(double* out, hipStream_t stream) override {
+    float host[4];
+    GEO_HIP(hipMemcpyAsync(host, dyn->scale, sizeof(host), hipMemcpyDeviceToHost, stream));
+    GEO_HIP(hipStreamSynchronize(stream));
+    for (int f = 0; f < 4; ++f) out[f] = host[f];
+    return GEO_OK;
+  }
+
+  int set_scales(const double* in, hipStream_t stream) override {
+    float host[8];
+    for (int f = 0; f < 4; ++f) {
+      if (!valid_scale(in[f]))
+        return fail(GEO_ERR_INVALID, "storage scales must be powers of two in [2^-60, 2^60]");
+      if (sizeof(T) != 2 && in[f] != 1.0)
+        return fail(GEO_ERR_INVALID, "only FP16/BF16 storage takes scales other than 1");
+      host[f] = (float)in[f];
+      host[4 + f] = (float)(1.0 / in[f]);
+    }
+    // scale[] and inv_scale[] are adjacent in the device block
+    GEO_HIP(hipMemcpyAsync(dyn->scale, host, sizeof(host), hipMemcpyHostToDevice, stream));
+    GEO_HIP(hipStreamSynchronize(stream));
     return GEO_OK;
   }
 };
@@ -1111,6 +1341,16 @@ int geo_diagnostics(GeoPlan* plan, GeoDiagOut* out, void* stream) {
 int geo_time_kernels(GeoPlan* plan, int64_t n_steps, void* stream, double* out_ms) {
   if (!plan || !out_ms) return fail(GEO_ERR_INVALID, "null argument");
   return plan->impl->time_kernels(n_steps, static_cast<hipStream_t>(stream), out_ms);
+}
+
+int geo_get_storage_scales(GeoPlan* plan, double* out4, void* stream) {
+  if (!plan || !out4) return fail(GEO_ERR_INVALID, "null argument");
+  return plan->impl->get_scales(out4, static_cast<hipStream_t>(stream));
+}
+
+int geo_set_storage_scales(GeoPlan* plan, const double* in4, void* stream) {
+  if (!plan || !in4) return fail(GEO_ERR_INVALID, "null argument");
+  return plan->impl->set_scales(in4, static_cast<hipStream_t>(stream));
 }
 
 void geo_plan_destroy(GeoPlan* plan) { delete plan; }

@@ -53,8 +53,19 @@ struct Dynamic {
   int64_t counter;        // steps completed since geo_step started
   int64_t trace_rows;     // rows in the trace buffer (0: no trace)
   void* trace;            // (trace_rows, n_receivers) compute type
-  const void* currents;   // (n,) compute type, or nullptr
+  const void* currents;   // (n_currents,) compute type, or nullptr
+  int64_t n_currents;
+  // Narrow storage: stored = value * scale, per field in the order er, et,
+  // hr, ht.  Powers of two, so scaling is exact; the rescale kernels change
+  // them between steps, never while a step kernel runs.
+  float scale[4];
+  float inv_scale[4];
+  float factor[4];        // pending exact rescale of the stored values
+  int32_t overflow;       // sticky: a non-finite stored value was seen
+  int32_t reserved;
 };
+
+enum : int32_t { kFieldEr = 0, kFieldEt = 1, kFieldHr = 2, kFieldHt = 3 };
 
 template <typename T> struct Strip {
   static constexpr int N = 16 / sizeof(T);
@@ -115,6 +126,32 @@ template <typename T, typename C>
 __device__ __forceinline__ void store_one(T* __restrict__ p, C x, C scale) {
   if constexpr (is_narrow<T>()) *p = from_c<T>(x * scale);
   else *p = from_c<T>(x);
+}
+
+// Storage scales of the four fields as seen by one kernel launch: read once
+// from the device block for narrow types, identically 1 otherwise.
+template <typename C>
+struct Scales {
+  C er, et, hr, ht, er_inv, et_inv, hr_inv, ht_inv;
+};
+
+template <typename T, typename C>
+__device__ __forceinline__ Scales<C> load_scales(const Dynamic* dyn) {
+  Scales<C> s;
+  if constexpr (is_narrow<T>()) {
+    s.er = dyn->scale[kFieldEr];
+    s.et = dyn->scale[kFieldEt];
+    s.hr = dyn->scale[kFieldHr];
+    s.ht = dyn->scale[kFieldHt];
+    s.er_inv = dyn->inv_scale[kFieldEr];
+    s.et_inv = dyn->inv_scale[kFieldEt];
+    s.hr_inv = dyn->inv_scale[kFieldHr];
+    s.ht_inv = dyn->inv_scale[kFieldHt];
+  } else {
+    s.er = s.et = s.hr = s.ht = (C)1;
+    s.er_inv = s.et_inv = s.hr_inv = s.ht_inv = (C)1;
+  }
+  return s;
 }
 
 // Aligned loads / stores of M compute-type values (16-byte multiples): the
@@ -193,7 +230,6 @@ struct HParams {
   int32_t pn, pm;               // node / midpoint row pitch
   int32_t R;
   int32_t si_on;
-  C s_er_inv, s_et_inv, s_ht, s_ht_inv, s_hr, s_hr_inv;
   const int32_t* __restrict__ edges;      // (E,2)
   const C* __restrict__ edge_grad;        // (E,) dt/(mu0 theta)
   const int32_t* __restrict__ face_edges; // (F,3)
@@ -235,7 +271,6 @@ struct EParams {
   const T* __restrict__ hr;
   int32_t pn, pm, R;
   int32_t ca_er_stride, ca_et_stride;     // 0 or row pitch
-  C s_er, s_er_inv, s_et, s_et_inv, s_ht_inv, s_hr_inv;
   const int32_t* __restrict__ vertex_order;  // (V,) processing order
   const int32_t* __restrict__ vertex_edges; // (V,8) padded
   const C* __restrict__ vertex_w;           // (V,8) sign*phi/A, 0 on padding
@@ -267,25 +302,26 @@ struct EParams {
 // ---------------------------------------------------------------------------
 
 template <typename T, typename C>
-__device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, int32_t e, int32_t strip) {
+__device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, const Scales<C>& sc, int32_t e,
+                                             int32_t strip) {
   constexpr int N = Strip<T>::N;
   const int32_t k0 = strip * N;
   const int32_t tail = p.edges[2 * e];
   const int32_t head = p.edges[2 * e + 1];
   const C a = p.edge_grad[e];
   C ert[N], erh[N], ht[N], et[N];
-  load_strip(p.er + (size_t)tail * p.pn + k0, ert, p.s_er_inv);
-  load_strip(p.er + (size_t)head * p.pn + k0, erh, p.s_er_inv);
+  load_strip(p.er + (size_t)tail * p.pn + k0, ert, sc.er_inv);
+  load_strip(p.er + (size_t)head * p.pn + k0, erh, sc.er_inv);
   T* hrow = p.ht + (size_t)e * p.pn + k0;
-  load_strip(hrow, ht, p.s_ht_inv);
+  load_strip(hrow, ht, sc.ht_inv);
   const T* erow = p.et + (size_t)e * p.pm;
   if (k0 < p.pm) {
-    load_strip(erow + k0, et, p.s_et_inv);
+    load_strip(erow + k0, et, sc.et_inv);
   } else {
 #pragma unroll
     for (int i = 0; i < N; ++i) et[i] = (C)0;
   }
-  C prev = (k0 > 0) ? load_one(erow + k0 - 1, p.s_et_inv) : (C)0;
+  C prev = (k0 > 0) ? load_one(erow + k0 - 1, sc.et_inv) : (C)0;
   C rb[N], rc[N], ir[N];
   load_vec<N>(p.rd_b + k0, rb);
   load_vec<N>(p.rd_c + k0, rc);
@@ -303,9 +339,9 @@ __device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, int32_t e, 
   if (p.si_on && k0 == 0) {
     // element 0 belongs to the surface-impedance segment
 #pragma unroll
-    for (int i = 1; i < N; ++i) store_one(hrow + i, ht[i], p.s_ht);
+    for (int i = 1; i < N; ++i) store_one(hrow + i, ht[i], sc.ht);
   } else {
-    store_strip(hrow, ht, p.s_ht);
+    store_strip(hrow, ht, sc.ht);
   }
 }
 
@@ -317,7 +353,8 @@ __device__ __forceinline__ void cross_b(const C* v, const C* b, C* out) {
 }
 
 template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
-__device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, int32_t f, int32_t strip) {
+__device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scales<C>& sc, int32_t f,
+                                             int32_t strip) {
   constexpr int N = Strip<T>::N;
   const int32_t j0 = strip * N;
   const int32_t e0 = p.face_edges[3 * f];
@@ -325,17 +362,17 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, int32_t f, 
   const int32_t e2 = p.face_edges[3 * f + 2];
   const C w0 = p.face_curl[3 * f], w1 = p.face_curl[3 * f + 1], w2 = p.face_curl[3 * f + 2];
   C et0[N], et1[N], et2[N], hr[N];
-  load_strip(p.et + (size_t)e0 * p.pm + j0, et0, p.s_et_inv);
-  load_strip(p.et + (size_t)e1 * p.pm + j0, et1, p.s_et_inv);
-  load_strip(p.et + (size_t)e2 * p.pm + j0, et2, p.s_et_inv);
+  load_strip(p.et + (size_t)e0 * p.pm + j0, et0, sc.et_inv);
+  load_strip(p.et + (size_t)e1 * p.pm + j0, et1, sc.et_inv);
+  load_strip(p.et + (size_t)e2 * p.pm + j0, et2, sc.et_inv);
   T* hrow = p.hr + (size_t)f * p.pm + j0;
-  load_strip(hrow, hr, p.s_hr_inv);
+  load_strip(hrow, hr, sc.hr_inv);
   C irb[N];
   load_vec<N>(p.inv_rbar + j0, irb);
 #pragma unroll
   for (int i = 0; i < N; ++i)  // inv_rbar is zero on padding lanes
     hr[i] = hr[i] - irb[i] * (w0 * et0[i] + w1 * et1[i] + w2 * et2[i]);
-  store_strip(hrow, hr, p.s_hr);
+  store_strip(hrow, hr, sc.hr);
 
   if constexpr (PLASMA) {
     const PlasmaTables<C>& pl = p.pl;
@@ -346,14 +383,14 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, int32_t f, 
     // (plasma.py:492-500).
     const int32_t v0 = p.faces[3 * f], v1 = p.faces[3 * f + 1], v2 = p.faces[3 * f + 2];
     C a0[N + 1], a1[N + 1], a2[N + 1];
-    load_strip(p.er + (size_t)v0 * p.pn + j0, a0, p.s_er_inv);
-    load_strip(p.er + (size_t)v1 * p.pn + j0, a1, p.s_er_inv);
-    load_strip(p.er + (size_t)v2 * p.pn + j0, a2, p.s_er_inv);
+    load_strip(p.er + (size_t)v0 * p.pn + j0, a0, sc.er_inv);
+    load_strip(p.er + (size_t)v1 * p.pn + j0, a1, sc.er_inv);
+    load_strip(p.er + (size_t)v2 * p.pn + j0, a2, sc.er_inv);
     // node j0+N is needed only while midpoint j0+N-1 < R, i.e. j0+N <= R
     const bool top = j0 + N <= p.R;
-    a0[N] = top ? load_one(p.er + (size_t)v0 * p.pn + j0 + N, p.s_er_inv) : (C)0;
-    a1[N] = top ? load_one(p.er + (size_t)v1 * p.pn + j0 + N, p.s_er_inv) : (C)0;
-    a2[N] = top ? load_one(p.er + (size_t)v2 * p.pn + j0 + N, p.s_er_inv) : (C)0;
+    a0[N] = top ? load_one(p.er + (size_t)v0 * p.pn + j0 + N, sc.er_inv) : (C)0;
+    a1[N] = top ? load_one(p.er + (size_t)v1 * p.pn + j0 + N, sc.er_inv) : (C)0;
+    a2[N] = top ? load_one(p.er + (size_t)v2 * p.pn + j0 + N, sc.er_inv) : (C)0;
     C rc[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) rc[q] = pl.recon[9 * f + q];
@@ -445,17 +482,17 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, int32_t f, 
 }
 
 template <typename T, typename C>
-__device__ __forceinline__ void h_si_edge(const HParams<T, C>& p, int32_t e) {
+__device__ __forceinline__ void h_si_edge(const HParams<T, C>& p, const Scales<C>& sc, int32_t e) {
   // Implicit lower-boundary Ht with 16-pole diffusive memory
   // (surface_impedance.py:225-270).
   const int32_t tail = p.edges[2 * e];
   const int32_t head = p.edges[2 * e + 1];
   const C forcing = p.edge_grad[e] * p.inv_r[0] *
-                    (load_one(p.er + (size_t)head * p.pn, p.s_er_inv) -
-                     load_one(p.er + (size_t)tail * p.pn, p.s_er_inv));
+                    (load_one(p.er + (size_t)head * p.pn, sc.er_inv) -
+                     load_one(p.er + (size_t)tail * p.pn, sc.er_inv));
   T* hp = p.ht + (size_t)e * p.pn;
-  const C h_old = load_one(hp, p.s_ht_inv);
-  const C et0 = load_one(p.et + (size_t)e * p.pm, p.s_et_inv);
+  const C h_old = load_one(hp, sc.ht_inv);
+  const C et0 = load_one(p.et + (size_t)e * p.pm, sc.et_inv);
   const C scale = p.si_scale[e * p.si_scale_stride];
   C* mem = p.si_mem + e;
   const size_t ts = (size_t)p.n_edges;
@@ -472,7 +509,7 @@ __device__ __forceinline__ void h_si_edge(const HParams<T, C>& p, int32_t e) {
     const C h_new = ((C)1 - (C)0.5 * coupling) * h_old + forcing - p.si_cs_metric * et0 +
                     p.si_cs * hist;
     const C h = h_new / ((C)1 + (C)0.5 * coupling);
-    store_one(hp, h, p.s_ht);
+    store_one(hp, h, sc.ht);
     const C sum = h + h_old;
 #pragma unroll
     for (int i = 0; i < kSiFastTerms; ++i)
@@ -484,7 +521,7 @@ __device__ __forceinline__ void h_si_edge(const HParams<T, C>& p, int32_t e) {
     const C h_new = ((C)1 - (C)0.5 * coupling) * h_old + forcing - p.si_cs_metric * et0 +
                     p.si_cs * hist;
     const C h = h_new / ((C)1 + (C)0.5 * coupling);
-    store_one(hp, h, p.s_ht);
+    store_one(hp, h, sc.ht);
     const C sum = h + h_old;
     for (int i = 0; i < p.si_terms; ++i)
       mem[i * ts] = mem[i * ts] * p.si_decay[i] + sum * p.si_drive[i];
@@ -492,7 +529,7 @@ __device__ __forceinline__ void h_si_edge(const HParams<T, C>& p, int32_t e) {
 }
 
 template <typename T, typename C>
-__device__ __forceinline__ void sample_block(const HParams<T, C>& p) {
+__device__ __forceinline__ void sample_block(const HParams<T, C>& p, const Scales<C>& sc) {
   // Receiver row for the Er state entering this step, then advance the
   // device step counter (only this block reads or writes it in the H kernel).
   __shared__ int64_t row;
@@ -506,7 +543,7 @@ __device__ __forceinline__ void sample_block(const HParams<T, C>& p) {
       C acc = (C)0;
 #pragma unroll
       for (int q = 0; q < 3; ++q)
-        acc += load_one(p.er + (size_t)p.rx_vertex[3 * i + q] * p.pn + layer, p.s_er_inv) *
+        acc += load_one(p.er + (size_t)p.rx_vertex[3 * i + q] * p.pn + layer, sc.er_inv) *
                p.rx_w[3 * i + q];
       trace[r * p.n_rx + i] = acc;
     }
@@ -519,22 +556,23 @@ template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
 __global__ void __launch_bounds__(kBlock) h_step_kernel(const HParams<T, C> p) {
   const int32_t L = logical_block(p.per_xcd);
   if (L >= p.n_blocks) return;
+  const Scales<C> sc = load_scales<T, C>(p.dyn);
   const uint32_t m = p.block_map[L];
   const uint32_t seg = m >> 24;
   const int32_t b = (int32_t)(m & 0xffffffu);
   if (seg == kSegHEdge) {
     int32_t row, strip;
     row_and_strip(p.g_edge, b, &row, &strip);
-    if (row < p.g_edge.n_rows) h_edge_strip(p, row, strip);
+    if (row < p.g_edge.n_rows) h_edge_strip(p, sc, row, strip);
   } else if (seg == kSegHFace) {
     int32_t row, strip;
     row_and_strip(p.g_face, b, &row, &strip);
-    if (row < p.g_face.n_rows) h_face_strip<T, C, PLASMA, MAGNETIZED>(p, row, strip);
+    if (row < p.g_face.n_rows) h_face_strip<T, C, PLASMA, MAGNETIZED>(p, sc, row, strip);
   } else if (seg == kSegSi) {
     const int32_t e = b * kBlock + (int32_t)threadIdx.x;
-    if (e < p.n_edges) h_si_edge(p, e);
+    if (e < p.n_edges) h_si_edge(p, sc, e);
   } else {
-    sample_block(p);
+    sample_block(p, sc);
   }
 }
 
@@ -600,8 +638,8 @@ __device__ __forceinline__ void plasma_project(const PlasmaTables<C>& pl, int32_
 }
 
 template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
-__device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, int32_t row,
-                                               int32_t strip) {
+__device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Scales<C>& sc,
+                                               int32_t row, int32_t strip) {
   constexpr int N = Strip<T>::N;
   const int32_t v = p.vertex_order[row];
   const int32_t k0 = strip * N;
@@ -616,13 +654,13 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, int32_t r
 #pragma unroll
   for (int s = 0; s < 6; ++s) {  // padding slots carry weight 0 and a valid edge
     C h[N];
-    load_strip(p.ht + (size_t)ve[s] * p.pn + k0, h, p.s_ht_inv);
+    load_strip(p.ht + (size_t)ve[s] * p.pn + k0, h, sc.ht_inv);
 #pragma unroll
     for (int i = 0; i < N; ++i) curl[i] += vw[s] * h[i];
   }
   T* erow = p.er + (size_t)v * p.pn + k0;
   C er[N], ca[N], cb[N], ir[N];
-  load_strip(erow, er, p.s_er_inv);
+  load_strip(erow, er, sc.er_inv);
   load_vec<N>(p.ca_er + (size_t)v * p.ca_er_stride + k0, ca);
   load_vec<N>(p.cb_er + (size_t)v * p.ca_er_stride + k0, cb);
   load_vec<N>(p.inv_r + k0, ir);
@@ -680,24 +718,25 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, int32_t r
   // ca/cb are zero on padding lanes, which therefore stay exactly zero.
 #pragma unroll
   for (int i = 0; i < N; ++i) er[i] = ca[i] * er[i] + cb[i] * (ir[i] * curl[i] - jr[i] - src[i]);
-  store_strip(erow, er, p.s_er);
+  store_strip(erow, er, sc.er);
 }
 
 template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
-__device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, int32_t e, int32_t strip) {
+__device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scales<C>& sc, int32_t e,
+                                             int32_t strip) {
   constexpr int N = Strip<T>::N;
   const int32_t j0 = strip * N;
   const int32_t fl = p.edge_lr[2 * e];
   const int32_t fr = p.edge_lr[2 * e + 1];
   const C inv_phi = p.edge_dual[e];
   C hl[N], hrr[N], ht[N + 1], et[N];
-  load_strip(p.hr + (size_t)fl * p.pm + j0, hl, p.s_hr_inv);
-  load_strip(p.hr + (size_t)fr * p.pm + j0, hrr, p.s_hr_inv);
+  load_strip(p.hr + (size_t)fl * p.pm + j0, hl, sc.hr_inv);
+  load_strip(p.hr + (size_t)fr * p.pm + j0, hrr, sc.hr_inv);
   const T* hrow = p.ht + (size_t)e * p.pn;
-  load_strip(hrow + j0, ht, p.s_ht_inv);
-  ht[N] = (j0 + N <= p.R) ? load_one(hrow + j0 + N, p.s_ht_inv) : (C)0;
+  load_strip(hrow + j0, ht, sc.ht_inv);
+  ht[N] = (j0 + N <= p.R) ? load_one(hrow + j0 + N, sc.ht_inv) : (C)0;
   T* erow = p.et + (size_t)e * p.pm + j0;
-  load_strip(erow, et, p.s_et_inv);
+  load_strip(erow, et, sc.et_inv);
   C ca[N], cb[N], irb[N], ra[N], rcc[N];
   load_vec<N>(p.ca_et + (size_t)e * p.ca_et_stride + j0, ca);
   load_vec<N>(p.cb_et + (size_t)e * p.ca_et_stride + j0, cb);
@@ -745,23 +784,24 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, int32_t e, 
     const C rd = ra[i] * (ht[i + 1] - ht[i]) + rcc[i] * ht[i + 1];
     et[i] = ca[i] * et[i] + cb[i] * (grad - rd - jt[i] - src[i]);
   }
-  store_strip(erow, et, p.s_et);
+  store_strip(erow, et, sc.et);
 }
 
 template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
 __global__ void __launch_bounds__(kBlock) e_step_kernel(const EParams<T, C> p) {
   const int32_t L = logical_block(p.per_xcd);
   if (L >= p.n_blocks) return;
+  const Scales<C> sc = load_scales<T, C>(p.dyn);
   const uint32_t m = p.block_map[L];
   const uint32_t seg = m >> 24;
   const int32_t b = (int32_t)(m & 0xffffffu);
   int32_t row, strip;
   if (seg == kSegEVertex) {
     row_and_strip(p.g_vertex, b, &row, &strip);
-    if (row < p.g_vertex.n_rows) e_vertex_strip<T, C, PLASMA, MAGNETIZED>(p, row, strip);
+    if (row < p.g_vertex.n_rows) e_vertex_strip<T, C, PLASMA, MAGNETIZED>(p, sc, row, strip);
   } else {
     row_and_strip(p.g_edge, b, &row, &strip);
-    if (row < p.g_edge.n_rows) e_edge_strip<T, C, PLASMA, MAGNETIZED>(p, row, strip);
+    if (row < p.g_edge.n_rows) e_edge_strip<T, C, PLASMA, MAGNETIZED>(p, sc, row, strip);
   }
 }
 
@@ -770,20 +810,22 @@ __global__ void __launch_bounds__(kBlock) e_step_kernel(const EParams<T, C> p) {
 // ---------------------------------------------------------------------------
 
 __global__ void set_dynamic_kernel(Dynamic* dyn, int64_t counter, void* trace, int64_t rows,
-                                   const void* currents) {
+                                   const void* currents, int64_t n_currents) {
   dyn->counter = counter;
   dyn->trace = trace;
   dyn->trace_rows = rows;
   dyn->currents = currents;
+  dyn->n_currents = currents != nullptr ? n_currents : 0;
 }
 
 template <typename T, typename C>
-__global__ void sample_kernel(const T* __restrict__ er, int32_t pn, C s_er_inv,
+__global__ void sample_kernel(const T* __restrict__ er, int32_t pn, const Dynamic* dyn,
                               const int32_t* __restrict__ rx_vertex,
                               const int32_t* __restrict__ rx_layer, const C* __restrict__ rx_w,
                               int32_t n_rx, C* __restrict__ out) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n_rx) return;
+  const C s_er_inv = load_scales<T, C>(dyn).er_inv;
   const int32_t layer = rx_layer[i];
   C acc = (C)0;
 #pragma unroll

@@ -37,6 +37,19 @@ from .sources import GaussianCurrent, TangentialGaussianCurrent
 from .surface_impedance import ConductiveHalfSpaceSurface, surface_ade_tables
 
 DEFAULT_NARROW_SCALES = {"er": 1.0, "et": 1.0, "hr": 256.0, "ht": 256.0}
+_FIELD_ORDER = ("er", "et", "hr", "ht")
+
+
+def _is_scale(value) -> bool:
+    """A power of two in [2^-60, 2^60] (exact to multiply by in every storage type)."""
+    try:
+        v = float(value)
+    except (TypeError, ValueError):
+        return False
+    if not (v > 0.0 and np.isfinite(v)):
+        return False
+    mant, exp = np.frexp(v)
+    return mant == 0.5 and -60 <= exp - 1 <= 60
 
 
 @dataclass(frozen=True, slots=True)
@@ -173,6 +186,7 @@ class GeodesicFDTD:
         compile_chunk_size: int = 8,
         torch_threads: int | None = None,
         storage_scales: dict[str, float] | None = None,
+        storage_scaling: str = "auto",
     ) -> None:
         if (isinstance(compile_chunk_size, bool)
                 or not isinstance(compile_chunk_size, (int, np.integer))
@@ -254,8 +268,21 @@ class GeodesicFDTD:
         elif self.source is not None:
             self._source_distribution = self.source.staggered_distribution(self)
         scales = dict(DEFAULT_NARROW_SCALES)
+        unknown = set(storage_scales or {}) - set(scales)
+        if unknown:
+            raise ValueError(f"unknown storage scale fields: {sorted(unknown)}")
         scales.update(storage_scales or {})
-        self.storage_scales = scales
+        for name, value in scales.items():
+            if not _is_scale(value):
+                raise ValueError(f"storage scale for {name} must be a power of two in "
+                                 "[2^-60, 2^60]")
+        if storage_scaling not in ("auto", "dynamic", "static"):
+            raise ValueError("storage_scaling must be 'auto', 'dynamic' or 'static'")
+        if storage_scaling == "auto":
+            # FP16 needs its range managed; BF16 has the exponent range of FP32
+            storage_scaling = "dynamic" if self.backend.dtype_name == "float16" else "static"
+        self.storage_scaling = storage_scaling
+        self._initial_scales = scales
         self.time_s = 0.0
         self.steps = 0
         self.compiled = bool(compile_step)
@@ -354,10 +381,11 @@ class GeodesicFDTD:
             ca_er=f64(ca_er), cb_er=f64(cb_er), er_coef_rows=ca_er.shape[0],
             ca_et=f64(ca_et), cb_et=f64(cb_et), et_coef_rows=ca_et.shape[0],
             eps_er=f64(eps_er), eps_et=f64(eps_et),
-            scale_er=self.storage_scales["er"] if narrow else 1.0,
-            scale_et=self.storage_scales["et"] if narrow else 1.0,
-            scale_hr=self.storage_scales["hr"] if narrow else 1.0,
-            scale_ht=self.storage_scales["ht"] if narrow else 1.0)
+            scaling=int(narrow and self.storage_scaling == "dynamic"),
+            scale_er=self._initial_scales["er"] if narrow else 1.0,
+            scale_et=self._initial_scales["et"] if narrow else 1.0,
+            scale_hr=self._initial_scales["hr"] if narrow else 1.0,
+            scale_ht=self._initial_scales["ht"] if narrow else 1.0)
         si_terms = 0
         if self.surface_impedance is not None:
             tables = surface_ade_tables(self.surface_impedance, mesh.n_edges, self.time_step_s)
@@ -451,11 +479,23 @@ class GeodesicFDTD:
             self._plan = None
 
     # ------------------------------------------------------------ field views
+    @property
+    def storage_scales(self) -> dict[str, float]:
+        """Current storage scales (stored value = field * scale); all 1 for
+        FP32/FP64 storage.  With dynamic scaling they change between chunks."""
+        out = (C.c_double * 4)()
+        _native.check(self._lib.geo_get_storage_scales(self._plan, out, self._stream()),
+                      self._lib)
+        return dict(zip(_FIELD_ORDER, (float(x) for x in out)))
+
+    def _narrow(self) -> bool:
+        return self.backend.dtype_name in ("float16", "bfloat16")
+
     def _field_view(self, name: str):
         buf = getattr(self, f"_{name}_buf")
         cols = len(self.radii_m) if name in ("er", "ht") else len(self.radial_midpoints_m)
         view = buf[:, :cols]
-        if self.backend.dtype_name in ("float16", "bfloat16"):
+        if self._narrow():
             return view.float() / self.storage_scales[name]
         return view
 
@@ -465,8 +505,26 @@ class GeodesicFDTD:
         data = self._tensor(values, self.backend.compute_dtype)
         if tuple(data.shape) != (buf.shape[0], cols):
             raise ValueError(f"{name} must have shape {(buf.shape[0], cols)}")
-        if self.backend.dtype_name in ("float16", "bfloat16"):
-            data = data * self.storage_scales[name]
+        if self._narrow():
+            scales = self.storage_scales
+            peak = float(data.abs().max()) if data.numel() else 0.0
+            if not np.isfinite(peak):
+                raise ValueError(f"{name} values must be finite")
+            if self.storage_scaling == "dynamic" and peak > 0.0:
+                # place the peak at 2^10, as the device rescale does; the next
+                # rescale point couples it to the other fields
+                exponent = int(np.clip(np.floor(np.log2(1024.0 / peak)), -60, 60))
+                scales[name] = float(2.0 ** exponent)
+            limit = 65504.0 if self.backend.dtype_name == "float16" else 3.0e38
+            if peak * scales[name] >= limit:
+                raise ValueError(f"{name} values overflow {self.backend.dtype_name} storage at "
+                                 f"scale {scales[name]:g}")
+            data = data * scales[name]
+            buf[:, :cols].copy_(data)
+            arr = (C.c_double * 4)(*(scales[k] for k in _FIELD_ORDER))
+            _native.check(self._lib.geo_set_storage_scales(self._plan, arr, self._stream()),
+                          self._lib)
+            return
         buf[:, :cols].copy_(data)
 
     er = property(lambda self: self._field_view("er"),
@@ -731,7 +789,8 @@ class GeodesicFDTD:
             "max_abs_hr_a_m": d["max_abs_hr"],
             "max_abs_ht_a_m": d["max_abs_ht"],
             "energy_j": d["energy_er"] + d["energy_et"] + d["energy_hr"] + d["energy_ht"],
-            "field_finite": d["nonfinite"] == 0,
+            "field_finite": d["nonfinite"] == 0 and d["storage_overflow"] == 0,
+            "storage_scaling": self.storage_scaling if self._narrow() else "none",
         }
 
     @property

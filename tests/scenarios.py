@@ -14,6 +14,11 @@ Workloads (sizes as in the benchmark definitions):
                 n_e = 1.43e13 exp(-0.15 h') exp((beta-0.15)(z-h')) m^-3,
                 h' = 74 km, beta = 0.3 /km; nu = 1.8e11 exp(-0.15 z[km]) /s;
                 32 receivers drawn area-uniformly with seed 20260901.
+  * ``c3``   -- s8 x 96 cells with a 2:1-refined band over 60-90 km
+                (44 cells of 1.364 km below, 44 of 0.682 km in the band, 8 of
+                1.25 km above), SI ground, the c1 plasma and receivers; the
+                precision sweep runs it with FP64/FP32/FP16/BF16 storage.
+                ``c3_small`` is the same grid at s3 for oracle comparisons.
 """
 
 from __future__ import annotations
@@ -51,6 +56,13 @@ def drude_profile(midpoint_altitudes_m, h_prime_km=74.0, beta_per_km=0.3):
     return density, collision
 
 
+def refined_band_altitudes() -> tuple[float, ...]:
+    """96 radial cells, 0-100 km, 2:1 refinement over 60-90 km (balanced 2:1)."""
+    nodes = np.concatenate((np.linspace(0.0, 60e3, 45), np.linspace(60e3, 90e3, 45)[1:],
+                            np.linspace(90e3, 100e3, 9)[1:]))
+    return tuple(float(x) for x in nodes)
+
+
 def scenario(name: str, **overrides) -> dict:
     base = {
         "subdivision": 2, "radial_cells": 24, "minimum_altitude_m": 0.0,
@@ -63,7 +75,7 @@ def scenario(name: str, **overrides) -> dict:
         "source": {"latitude_deg": SOURCE_LAT, "longitude_deg": SOURCE_LON, "altitude_m": 0.0,
                    "carrier_frequency_hz": 20.0},
         "tangential": None, "dtype": "float64", "receivers": "c0", "time_step_s": None,
-        "seed_fields": False,
+        "seed_fields": False, "radial_altitudes_m": None,
     }
     if name == "c0":
         pass
@@ -112,6 +124,15 @@ def scenario(name: str, **overrides) -> dict:
         base.update(subdivision=7, radial_cells=64, dtype="float32", receivers="c1",
                     surface_sigma=None)
         base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3}
+    elif name in ("c3", "c3_small"):
+        alt = refined_band_altitudes()
+        base.update(subdivision=8 if name == "c3" else 3, radial_cells=len(alt) - 1,
+                    dtype="float32", receivers="c1", radial_altitudes_m=alt)
+        base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3}
+        if name == "c3_small":
+            # pulsed so that a short oracle run sees the source
+            base["source"] = dict(base["source"], one_over_e_half_width_s="30dt",
+                                  center_time_s="90dt")
     elif name in ("vacuum_s7_r64", "vacuum_s7_r80"):
         # the reference's benchmark workload: vacuum, PEC, seeded fields, no source
         base.update(subdivision=7, radial_cells=int(name[-2:]), dtype="float32",
@@ -181,7 +202,7 @@ def _plasma_arrays(sc, mesh, midpoint_altitudes):
 
 
 def build_sim(sc: dict, *, compile_step=False, compile_chunk_size=8, dtype=None,
-              storage_scales=None):
+              storage_scales=None, storage_scaling="auto"):
     import ionosphere_fdtd_amd as g
 
     cfg = g.SimulationConfig(
@@ -190,7 +211,8 @@ def build_sim(sc: dict, *, compile_step=False, compile_chunk_size=8, dtype=None,
         earth_radius_m=sc["earth_radius_m"], courant_factor=sc["courant_factor"],
         radial_boundary_condition="surface-impedance" if sc["surface_sigma"] else "pec",
         geometry_mode=sc["geometry_mode"], loss_integration=sc["loss_integration"],
-        time_step_s=sc["time_step_s"])
+        time_step_s=sc["time_step_s"], radial_altitudes_m=sc["radial_altitudes_m"],
+        radial_grid_policy="balanced-2to1" if sc["radial_altitudes_m"] else "smooth")
     material = VacuumMaterial() if sc["vacuum"] else g.EarthIonosphereMaterial(**sc["material"])
     surface = g.ConductiveHalfSpaceSurface(sc["surface_sigma"]) if sc["surface_sigma"] else None
     mesh = g.build_geodesic_mesh(sc["subdivision"])
@@ -202,7 +224,8 @@ def build_sim(sc: dict, *, compile_step=False, compile_chunk_size=8, dtype=None,
             mesh, alt, b, tuple(g.ColdPlasmaSpecies(*s) for s in species))
     kwargs = dict(material=material, surface_impedance=surface, plasma=plasma, mesh=mesh,
                   backend="hip", dtype=dtype or sc["dtype"], compile_step=compile_step,
-                  compile_chunk_size=compile_chunk_size, storage_scales=storage_scales)
+                  compile_chunk_size=compile_chunk_size, storage_scales=storage_scales,
+                  storage_scaling=storage_scaling)
     from ionosphere_fdtd_amd.solver import stable_time_step
 
     dt = stable_time_step(cfg, mesh, material, plasma)
@@ -217,7 +240,11 @@ def build_sim(sc: dict, *, compile_step=False, compile_chunk_size=8, dtype=None,
 
 
 def _midpoint_altitudes(sc):
-    alt = np.linspace(sc["minimum_altitude_m"], sc["maximum_altitude_m"], sc["radial_cells"] + 1)
+    if sc["radial_altitudes_m"] is not None:
+        alt = np.asarray(sc["radial_altitudes_m"], dtype=np.float64)
+    else:
+        alt = np.linspace(sc["minimum_altitude_m"], sc["maximum_altitude_m"],
+                          sc["radial_cells"] + 1)
     radii = sc["earth_radius_m"] + alt
     return 0.5 * (radii[:-1] + radii[1:]) - sc["earth_radius_m"]
 
@@ -240,6 +267,7 @@ def build_oracle(sc: dict, *, dtype=None, _dt_only=False):
                   maximum_altitude_m=sc["maximum_altitude_m"],
                   earth_radius_m=sc["earth_radius_m"], courant_factor=sc["courant_factor"],
                   time_step_s=sc["time_step_s"], loss_integration=sc["loss_integration"],
+                  radial_altitudes_m=sc["radial_altitudes_m"],
                   geometry_mode=sc["geometry_mode"], material=None if sc["vacuum"] else sc["material"],
                   material_arrays=material_arrays,
                   surface=o.OracleSurface(sc["surface_sigma"]) if sc["surface_sigma"] else None,
