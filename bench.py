@@ -129,7 +129,7 @@ def precision_sweep(args) -> dict:
             sc.update(source=None, seed_fields=True)
         ref_fields = ref_traces = None
         res = {}
-        for dtype in ("float64", "float32", "float16", "bfloat16"):
+        for dtype in ["float64"] + [d for d in args.sweep_dtypes.split(",") if d != "float64"]:
             t0 = time.perf_counter()
             sim = S.build_sim(sc, dtype=dtype, compile_step=True, compile_chunk_size=args.chunk)
             if sc["seed_fields"]:
@@ -186,6 +186,8 @@ def main() -> int:
                     help="run the FP64/FP32/FP16/BF16 storage sweep on --workload (c3: s8 x 96, "
                          "refined 60-90 km band) for --steps steps instead of the throughput "
                          "bench")
+    ap.add_argument("--sweep-dtypes", default="float32,float16,bfloat16",
+                    help="storage types compared with the FP64 run in --precision-sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the multi-GPU halo path is out of scope for this "

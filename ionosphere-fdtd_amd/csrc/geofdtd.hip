@@ -578,16 +578,17 @@ struct Plan final : PlanBase {
     const int64_t er_rows = ph->er_coef_rows, et_rows = ph->et_coef_rows;
     if (!(er_rows == 1 || er_rows == V) || !(et_rows == 1 || et_rows == E))
       return fail(GEO_ERR_INVALID, "coefficient rows must be 1 or the entity count");
-    std::vector<C> ca_er(er_rows * pn, 0), cb_er(er_rows * pn, 0), ca_et(et_rows * pm, 0),
+    // loss tables carry 1 - Ca (the kernels use the increment form)
+    std::vector<C> cl_er(er_rows * pn, 0), cb_er(er_rows * pn, 0), cl_et(et_rows * pm, 0),
         cb_et(et_rows * pm, 0);
     for (int64_t i = 0; i < er_rows; ++i)
       for (int64_t k = 0; k <= R; ++k) {
-        ca_er[i * pn + k] = (C)ph->ca_er[i * (R + 1) + k];
+        cl_er[i * pn + k] = (C)(1.0 - ph->ca_er[i * (R + 1) + k]);  // exact for Ca >= 1/2
         cb_er[i * pn + k] = (C)ph->cb_er[i * (R + 1) + k];
       }
     for (int64_t i = 0; i < et_rows; ++i)
       for (int64_t j = 0; j < R; ++j) {
-        ca_et[i * pm + j] = (C)ph->ca_et[i * R + j];
+        cl_et[i * pm + j] = (C)(1.0 - ph->ca_et[i * R + j]);
         cb_et[i * pm + j] = (C)ph->cb_et[i * R + j];
       }
     const bool narrow = sizeof(T) == 2;
@@ -687,8 +688,11 @@ struct Plan final : PlanBase {
         for (int q = 0; q < 6; ++q)
           for (int64_t i = 0; i < rows; ++i)
             for (int64_t j = band_lo; j < band_hi; ++j)
+              // slots 0 (exp(-nu dt)) and 1 (cos(omega_c dt)) are carried as
+              // 1 - value, exact in FP64 for values >= 1/2
               coef[((s * 6 + q) * rows + i) * ppitch + (j - band_lo)] =
-                  (C)ph->plasma_coef[((s * 6 + q) * rows + i) * R + j];
+                  (C)(q < 2 ? 1.0 - ph->plasma_coef[((s * 6 + q) * rows + i) * R + j]
+                            : ph->plasma_coef[((s * 6 + q) * rows + i) * R + j]);
       if (magnetized) {
         const int64_t brows = ph->plasma_b_rows;
         if (!(brows == 1 || brows == F)) return fail(GEO_ERR_INVALID, "B rows must be 1 or F");
@@ -712,8 +716,8 @@ struct Plan final : PlanBase {
     UP(edges, d_edges); UP(edge_lr, d_lr); UP(face_edges, d_fe); UP(faces, d_faces);
     UP(vedges, d_ve); UP(vorder, d_vo); UP(edge_grad, d_eg); UP(edge_dual, d_ed); UP(face_curl, d_fc);
     UP(vw, d_vw); UP(inv_r, d_ir); UP(rd_b, d_rdb); UP(rd_c, d_rdc); UP(inv_rbar, d_irb);
-    UP(rh_a, d_rha); UP(rh_c, d_rhc); UP(ca_er, d_caer); UP(cb_er, d_cber);
-    UP(ca_et, d_caet); UP(cb_et, d_cbet);
+    UP(rh_a, d_rha); UP(rh_c, d_rhc); UP(cl_er, d_caer); UP(cb_er, d_cber);
+    UP(cl_et, d_caet); UP(cb_et, d_cbet);
     if (si_on) UP(si_scale, d_sis);
     if (plasma) {
       UP(recon, d_recon); UP(center, d_center); UP(ltan, d_lt); UP(rtan, d_rt);
@@ -790,7 +794,7 @@ struct Plan final : PlanBase {
       hp.si_cs_metric = (C)(cs * metric);
       double hsum = 0.0;
       for (int i = 0; i < si_terms; ++i) {
-        hp.si_decay[i] = (C)ph->si_decay[i];
+        hp.si_loss[i] = (C)(1.0 - ph->si_decay[i]);
         hp.si_drive[i] = (C)ph->si_drive[i];
         hp.si_hw[i] = (C)ph->si_history_weights[i];
         hsum += (double)(C)ph->si_history_weights[i];
@@ -826,9 +830,9 @@ struct Plan final : PlanBase {
     ep.vertex_w = d_vw;
     ep.edge_lr = d_lr;
     ep.edge_dual = d_ed;
-    ep.ca_er = d_caer;
+    ep.cl_er = d_caer;
     ep.cb_er = d_cber;
-    ep.ca_et = d_caet;
+    ep.cl_et = d_caet;
     ep.cb_et = d_cbet;
     ep.inv_r = d_ir;
     ep.inv_rbar = d_irb;

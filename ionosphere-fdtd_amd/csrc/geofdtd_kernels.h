@@ -248,7 +248,8 @@ struct HParams {
   C si_cs;                                // 2 dt/(mu0 dr0)
   C si_cs_metric;                         // si_cs * rbar0/r0
   C si_hw_sum;                            // sum of history weights
-  C si_decay[kMaxSiTerms], si_drive[kMaxSiTerms], si_hw[kMaxSiTerms];
+  // si_loss = 1 - decay of each memory term (increment form, as the loss)
+  C si_loss[kMaxSiTerms], si_drive[kMaxSiTerms], si_hw[kMaxSiTerms];
   PlasmaTables<C> pl;
   // receivers + step counter
   Dynamic* dyn;
@@ -276,9 +277,9 @@ struct EParams {
   const C* __restrict__ vertex_w;           // (V,8) sign*phi/A, 0 on padding
   const int32_t* __restrict__ edge_lr;      // (E,2) left, right face
   const C* __restrict__ edge_dual;          // (E,) 1/phi
-  const C* __restrict__ ca_er;
+  const C* __restrict__ cl_er;            // 1 - Ca (loss), zero on padding
   const C* __restrict__ cb_er;
-  const C* __restrict__ ca_et;
+  const C* __restrict__ cl_et;
   const C* __restrict__ cb_et;
   const C* __restrict__ inv_r;            // (pn,)
   const C* __restrict__ inv_rbar;         // (pm,)
@@ -415,8 +416,9 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
       for (int s = 0; s < pl.n_species; ++s) {
         C* jp = pl.current + s * pl.species_stride + (size_t)f * pl.pitch + jb0;
         const C* cf = pl.coef + s * pl.coef_species_stride + (size_t)f * pl.coef_row_stride + jb0;
-        C decay[N], dre[N];
-        load_vec<N>(cf, decay);
+        // slot 0 holds 1 - exp(-nu dt): J + (d E - (1 - decay) J)
+        C loss[N], dre[N];
+        load_vec<N>(cf, loss);
         load_vec<N>(cf + 4 * pl.coef_q_stride, dre);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -424,7 +426,7 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
           load_vec<N>(jp + q * pl.comp_stride, x);
 #pragma unroll
           for (int i = 0; i < N; ++i)
-            x[i] = (j0 + i < pl.band_hi) ? decay[i] * x[i] + dre[i] * drive[q][i] : x[i];
+            x[i] = (j0 + i < pl.band_hi) ? x[i] + (dre[i] * drive[q][i] - loss[i] * x[i]) : x[i];
           store_vec<N>(jp + q * pl.comp_stride, x);
         }
       }
@@ -440,11 +442,12 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
       for (int s = 0; s < pl.n_species; ++s) {
         C* jp = pl.current + s * pl.species_stride + (size_t)f * pl.pitch + jb0;
         const C* cf = pl.coef + s * pl.coef_species_stride + (size_t)f * pl.coef_row_stride + jb0;
-        C J[3][N], decay[N], dre[N], cs[N], sn[N], dpar[N], dim[N];
+        // slots 0 and 1 hold 1 - exp(-nu dt) and 1 - cos(omega_c dt)
+        C J[3][N], loss[N], dre[N], vers[N], sn[N], dpar[N], dim[N];
 #pragma unroll
         for (int d = 0; d < 3; ++d) load_vec<N>(jp + d * pl.comp_stride, J[d]);
-        load_vec<N>(cf, decay);
-        load_vec<N>(cf + pl.coef_q_stride, cs);
+        load_vec<N>(cf, loss);
+        load_vec<N>(cf + pl.coef_q_stride, vers);
         load_vec<N>(cf + 2 * pl.coef_q_stride, sn);
         load_vec<N>(cf + 3 * pl.coef_q_stride, dpar);
         load_vec<N>(cf + 4 * pl.coef_q_stride, dre);
@@ -468,11 +471,14 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
           }
           cross_b(eperp, b, ecr);
           cross_b(jperp, b, jcr);
+          // decay (J_par + cos J_perp + sin J_perp x b) + drive terms, with
+          // the rotation written as J + rot, rot = sin (J_perp x b) - (1 - cos) J_perp
 #pragma unroll
-          for (int d = 0; d < 3; ++d)
-            J[d][i] = decay[i] * jpar[d] + dpar[i] * epar[d] +
-                      decay[i] * (cs[i] * jperp[d] + sn[i] * jcr[d]) + dre[i] * eperp[d] +
-                      dim[i] * ecr[d];
+          for (int d = 0; d < 3; ++d) {
+            const C rot = sn[i] * jcr[d] - vers[i] * jperp[d];
+            J[d][i] = Ji[d] + (rot - loss[i] * (Ji[d] + rot) + dpar[i] * epar[d] +
+                               dre[i] * eperp[d] + dim[i] * ecr[d]);
+          }
         }
 #pragma unroll
         for (int d = 0; d < 3; ++d) store_vec<N>(jp + d * pl.comp_stride, J[d]);
@@ -484,7 +490,10 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
 template <typename T, typename C>
 __device__ __forceinline__ void h_si_edge(const HParams<T, C>& p, const Scales<C>& sc, int32_t e) {
   // Implicit lower-boundary Ht with 16-pole diffusive memory
-  // (surface_impedance.py:225-270).
+  // (surface_impedance.py:225-270), in increment form:
+  // ((1 - c/2) h_old + rest) / (1 + c/2) = h_old + (rest - c h_old) / (1 + c/2)
+  // and m' = m + (sum drive - (1 - decay) m), so the small coupling c and the
+  // slow decay rates of the low-frequency poles keep their relative precision.
   const int32_t tail = p.edges[2 * e];
   const int32_t head = p.edges[2 * e + 1];
   const C forcing = p.edge_grad[e] * p.inv_r[0] *
@@ -506,25 +515,25 @@ __device__ __forceinline__ void h_si_edge(const HParams<T, C>& p, const Scales<C
 #pragma unroll
     for (int i = 0; i < kSiFastTerms; ++i) hist += m[i] * p.si_hw[i];
     hist *= scale;
-    const C h_new = ((C)1 - (C)0.5 * coupling) * h_old + forcing - p.si_cs_metric * et0 +
-                    p.si_cs * hist;
-    const C h = h_new / ((C)1 + (C)0.5 * coupling);
+    const C h = h_old + (forcing - p.si_cs_metric * et0 + p.si_cs * hist - coupling * h_old) /
+                            ((C)1 + (C)0.5 * coupling);
     store_one(hp, h, sc.ht);
     const C sum = h + h_old;
 #pragma unroll
     for (int i = 0; i < kSiFastTerms; ++i)
-      if (i < p.si_terms) mem[i * ts] = m[i] * p.si_decay[i] + sum * p.si_drive[i];
+      if (i < p.si_terms) mem[i * ts] = m[i] + (sum * p.si_drive[i] - p.si_loss[i] * m[i]);
   } else {
     C hist = (C)0;
     for (int i = 0; i < p.si_terms; ++i) hist += mem[i * ts] * p.si_hw[i];
     hist *= scale;
-    const C h_new = ((C)1 - (C)0.5 * coupling) * h_old + forcing - p.si_cs_metric * et0 +
-                    p.si_cs * hist;
-    const C h = h_new / ((C)1 + (C)0.5 * coupling);
+    const C h = h_old + (forcing - p.si_cs_metric * et0 + p.si_cs * hist - coupling * h_old) /
+                            ((C)1 + (C)0.5 * coupling);
     store_one(hp, h, sc.ht);
     const C sum = h + h_old;
-    for (int i = 0; i < p.si_terms; ++i)
-      mem[i * ts] = mem[i * ts] * p.si_decay[i] + sum * p.si_drive[i];
+    for (int i = 0; i < p.si_terms; ++i) {
+      const C mi = mem[i * ts];
+      mem[i * ts] = mi + (sum * p.si_drive[i] - p.si_loss[i] * mi);
+    }
   }
 }
 
@@ -659,9 +668,9 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Sca
     for (int i = 0; i < N; ++i) curl[i] += vw[s] * h[i];
   }
   T* erow = p.er + (size_t)v * p.pn + k0;
-  C er[N], ca[N], cb[N], ir[N];
+  C er[N], cl[N], cb[N], ir[N];
   load_strip(erow, er, sc.er_inv);
-  load_vec<N>(p.ca_er + (size_t)v * p.ca_er_stride + k0, ca);
+  load_vec<N>(p.cl_er + (size_t)v * p.ca_er_stride + k0, cl);
   load_vec<N>(p.cb_er + (size_t)v * p.ca_er_stride + k0, cb);
   load_vec<N>(p.inv_r + k0, ir);
   C jr[N];
@@ -715,9 +724,13 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Sca
         if (t == i) src[t] += value;
     }
   }
-  // ca/cb are zero on padding lanes, which therefore stay exactly zero.
+  // Exponential loss in increment form, Er + (Cb * drive - (1 - Ca) * Er):
+  // 1 - Ca = -expm1(-sigma dt / eps) keeps its relative precision where
+  // Ca rounds to within an ulp of 1 (sigma ~ 1e-12 S/m near the ground).
+  // cl/cb are zero on padding lanes, which therefore stay exactly zero.
 #pragma unroll
-  for (int i = 0; i < N; ++i) er[i] = ca[i] * er[i] + cb[i] * (ir[i] * curl[i] - jr[i] - src[i]);
+  for (int i = 0; i < N; ++i)
+    er[i] = er[i] + (cb[i] * (ir[i] * curl[i] - jr[i] - src[i]) - cl[i] * er[i]);
   store_strip(erow, er, sc.er);
 }
 
@@ -737,8 +750,8 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
   ht[N] = (j0 + N <= p.R) ? load_one(hrow + j0 + N, sc.ht_inv) : (C)0;
   T* erow = p.et + (size_t)e * p.pm + j0;
   load_strip(erow, et, sc.et_inv);
-  C ca[N], cb[N], irb[N], ra[N], rcc[N];
-  load_vec<N>(p.ca_et + (size_t)e * p.ca_et_stride + j0, ca);
+  C cl[N], cb[N], irb[N], ra[N], rcc[N];
+  load_vec<N>(p.cl_et + (size_t)e * p.ca_et_stride + j0, cl);
   load_vec<N>(p.cb_et + (size_t)e * p.ca_et_stride + j0, cb);
   load_vec<N>(p.inv_rbar + j0, irb);
   load_vec<N>(p.rh_a + j0, ra);
@@ -782,7 +795,7 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
   for (int i = 0; i < N; ++i) {
     const C grad = irb[i] * inv_phi * (hl[i] - hrr[i]);
     const C rd = ra[i] * (ht[i + 1] - ht[i]) + rcc[i] * ht[i + 1];
-    et[i] = ca[i] * et[i] + cb[i] * (grad - rd - jt[i] - src[i]);
+    et[i] = et[i] + (cb[i] * (grad - rd - jt[i] - src[i]) - cl[i] * et[i]);
   }
   store_strip(erow, et, sc.et);
 }
