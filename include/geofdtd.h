@@ -228,6 +228,29 @@ void geo_plan_destroy(GeoPlan* plan);
  * FETCH_SIZE / WRITE_SIZE readings calibrate the counters. */
 int geo_stream_probe(void* dst, const void* src, int64_t bytes, int32_t mode, int32_t repeats,
                      void* stream, double* out_ms);
+/* Ensembles (no reference counterpart: the reference advances one
+ * GeodesicFDTD per process, solver.py:789-816).  Independent runs on the
+ * same mesh -- each a bound plan with its own radial grid, time step,
+ * coefficients, plasma, ground, source and receivers -- advance together:
+ * one H and one E launch per step cover every run, full chunks replay as one
+ * HIP graph.  Plans must outlive the ensemble and share the storage type and
+ * plasma kind; dynamically scaled FP16 plans are not accepted.  Results are
+ * bitwise those of stepping every plan alone. */
+typedef struct GeoEnsemble GeoEnsemble;
+int geo_ensemble_create(GeoPlan* const* plans, int32_t n_plans, GeoEnsemble** out);
+/* Column-block shape: strips_per_thread 1, 2 or 4 (16-byte strips a thread
+ * updates per row); block_order 0 run-major, 1 runs interleaved. */
+int geo_ensemble_configure(GeoEnsemble* ens, int32_t strips_per_thread, int32_t block_order);
+/* Autotuner: time every shape over `steps` source-free steps, keep the
+ * fastest and restore every run's state.  Synchronises the stream. */
+int geo_ensemble_tune(GeoEnsemble* ens, int32_t steps, void* stream, int32_t* strips_per_thread,
+                      int32_t* block_order, double* ms_per_step);
+/* Advance every run n_steps.  currents: (n_plans, n_steps) compute type or
+ * NULL; traces: NULL or (n_plans, n_steps + 1, n_receivers) with the same
+ * receiver count in every run (geo_set_receivers on each plan). */
+int geo_ensemble_step(GeoEnsemble* ens, const void* currents, int64_t n_steps, int32_t chunk,
+                      void* traces, void* stream);
+void geo_ensemble_destroy(GeoEnsemble* ens);
 const char* geo_last_error(void);
 int geo_version(void);
 

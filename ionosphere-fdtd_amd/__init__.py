@@ -10,6 +10,7 @@ gfx950 kernels in ``libgeofdtd.so`` (C ABI in include/geofdtd.h).
 
 from .backends import ArrayBackend, BackendUnavailableError, create_backend
 from .constants import C_0, EARTH_RADIUS_M, EPSILON_0, MU_0
+from .ensemble import EnsembleFDTD
 from .materials import EarthIonosphereMaterial
 from .mesh import GeodesicMesh, build_geodesic_mesh, build_geodesic_mesh_from_topology
 from .plasma import ColdPlasmaSpecies, MeshPlasmaModel, electron_drude_model
@@ -22,6 +23,7 @@ from .surface_impedance import ConductiveHalfSpaceSurface
 __all__ = [
     "ArrayBackend", "BackendUnavailableError", "C_0", "ColdPlasmaSpecies",
     "ConductiveHalfSpaceSurface", "EARTH_RADIUS_M", "EPSILON_0", "EarthIonosphereMaterial",
+    "EnsembleFDTD",
     "GaussianCurrent", "GeodesicFDTD", "GeodesicMesh", "MU_0", "MeshPlasmaModel",
     "RadialGridValidation", "RadialRefinementRegion", "SimulationConfig",
     "TangentialGaussianCurrent", "build_geodesic_mesh", "build_geodesic_mesh_from_topology",

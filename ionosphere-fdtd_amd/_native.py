@@ -98,7 +98,8 @@ EXPORTED_SYMBOLS = (
     "geo_plan_create", "geo_plan_layout", "geo_plan_bind", "geo_step", "geo_set_receivers",
     "geo_sample_er", "geo_diagnostics", "geo_time_kernels", "geo_plan_destroy",
     "geo_last_error", "geo_version", "geo_stream_probe", "geo_get_storage_scales",
-    "geo_set_storage_scales",
+    "geo_set_storage_scales", "geo_ensemble_create", "geo_ensemble_configure",
+    "geo_ensemble_tune", "geo_ensemble_step", "geo_ensemble_destroy",
 )
 
 _lib = None
@@ -129,11 +130,19 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.geo_set_storage_scales.argtypes = [vp, _p_f64, vp]
     lib.geo_plan_destroy.argtypes = [vp]
     lib.geo_plan_destroy.restype = None
+    lib.geo_ensemble_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+    lib.geo_ensemble_configure.argtypes = [vp, C.c_int32, C.c_int32]
+    lib.geo_ensemble_tune.argtypes = [vp, C.c_int32, vp, C.POINTER(C.c_int32),
+                                      C.POINTER(C.c_int32), _p_f64]
+    lib.geo_ensemble_step.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp]
+    lib.geo_ensemble_destroy.argtypes = [vp]
+    lib.geo_ensemble_destroy.restype = None
     lib.geo_last_error.restype = C.c_char_p
     lib.geo_last_error.argtypes = []
     lib.geo_version.restype = C.c_int
     for name in EXPORTED_SYMBOLS:
-        if name not in ("geo_plan_destroy", "geo_last_error", "geo_version"):
+        if name not in ("geo_plan_destroy", "geo_last_error", "geo_version",
+                        "geo_ensemble_destroy"):
             getattr(lib, name).restype = C.c_int
     if path is None:
         _lib = lib

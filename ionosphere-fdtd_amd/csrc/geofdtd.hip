@@ -98,10 +98,13 @@ std::vector<uint32_t> build_block_map(const std::vector<SegSpec>& segs) {
   return map;
 }
 
-geo::SegGeom make_geom(int64_t rows, int64_t pitch, int64_t strip_elems) {
+// Rows of `pitch` elements in 16-byte strips of `strip_elems`, S strips per
+// thread; an integer number of rows per 256-thread block.
+geo::SegGeom make_geom(int64_t rows, int64_t pitch, int64_t strip_elems, int32_t S = 1) {
   geo::SegGeom g;
   g.n_rows = (int32_t)rows;
-  g.strips = (int32_t)(pitch / strip_elems);
+  g.row_strips = (int32_t)(pitch / strip_elems);
+  g.strips = (g.row_strips + S - 1) / S;
   g.rows_per_block = geo::kBlock / g.strips;
   g.magic = (uint32_t)((65536 + g.strips - 1) / g.strips);
   return g;
@@ -123,6 +126,7 @@ struct PlanBase {
   virtual int get_scales(double* out, hipStream_t stream) = 0;
   virtual int set_scales(const double* in, hipStream_t stream) = 0;
   GeoLayout layout{};
+  uint64_t generation = 0;  // advanced whenever kernel parameters change (bind, receivers)
 };
 
 // FP64 reduction kernels for diagnostics: per-block partials, then one block.
@@ -481,6 +485,8 @@ struct Plan final : PlanBase {
       return fail(GEO_ERR_INVALID, "vertex degree must be at most 6");
     pn = round_up(R + 1, N);
     pm = round_up(R, N);
+    if (pn / N > geo::kBlock)
+      return fail(GEO_ERR_INVALID, "radial columns longer than 256 strips are not supported");
     const double dt = ph->time_step_s;
     if (!(dt > 0.0) || !std::isfinite(dt))
       return fail(GEO_ERR_INVALID, "time step must be finite and positive");
@@ -961,6 +967,7 @@ struct Plan final : PlanBase {
     ep.hr = static_cast<const T*>(s->hr);
     ep.pl.current = static_cast<C*>(s->plasma_current);
     bound = true;
+    ++generation;
     return GEO_OK;
   }
 
@@ -997,6 +1004,7 @@ struct Plan final : PlanBase {
     hp.rx_w = dw;
     hp.n_rx = n;
     n_rx = n;
+    ++generation;
     return GEO_OK;
   }
 
@@ -1241,6 +1249,353 @@ __global__ void __launch_bounds__(geo::kBlock) stream_probe_kernel(f4v* __restri
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Ensembles of independent runs on one mesh (uneven radial grids, own
+// coefficients, plasma, ground, sources and receivers), advanced by one H and
+// one E launch per step for all runs, full chunks replayed as one HIP graph.
+//
+// Column-block shape (the tuned parameters):
+//   S       strips per thread (1, 2 or 4): a thread updates S 16-byte strips
+//           of one row, amortising the row's index and metric loads;
+//   policy  block order: 0 run-major (each XCD sweeps whole runs, so a run's
+//           neighbour columns stay in one L2), 1 runs interleaved block by
+//           block (every XCD works on every run).
+// ---------------------------------------------------------------------------
+struct EnsembleBase {
+  virtual ~EnsembleBase() = default;
+  virtual int configure(int32_t S, int32_t policy) = 0;
+  virtual int step(const void* currents, int64_t n, int32_t chunk, void* traces,
+                   hipStream_t stream) = 0;
+  virtual int tune(int32_t steps, hipStream_t stream, int32_t* best_s, int32_t* best_policy,
+                   double* best_ms) = 0;
+  int32_t n_runs = 0;
+  int32_t S = 1, policy = 0;
+};
+
+template <typename T, typename C>
+struct Ensemble final : EnsembleBase {
+  static constexpr int N = geo::Strip<T>::N;
+  std::vector<Plan<T, C>*> plans;
+  std::vector<uint64_t> seen;
+  bool plasma = false, magnetized = false;
+  geo::HParams<T, C>* d_hp = nullptr;
+  geo::EParams<T, C>* d_ep = nullptr;
+  geo::Dynamic** d_dyns = nullptr;
+  uint32_t* d_hmap = nullptr;
+  uint32_t* d_emap = nullptr;
+  int32_t h_blocks = 0, e_blocks = 0, h_per_xcd = 0, e_per_xcd = 0;
+  std::vector<geo::HParams<T, C>> host_hp;
+  std::vector<geo::EParams<T, C>> host_ep;
+  hipStream_t capture_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  int32_t graph_chunk = 0;
+
+  ~Ensemble() override {
+    drop_graph();
+    for (void* p : {(void*)d_hp, (void*)d_ep, (void*)d_dyns, (void*)d_hmap, (void*)d_emap})
+      if (p) (void)hipFree(p);
+    if (capture_stream) (void)hipStreamDestroy(capture_stream);
+  }
+
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_chunk = 0;
+  }
+
+  int create(const std::vector<Plan<T, C>*>& ps) {
+    plans = ps;
+    n_runs = (int32_t)ps.size();
+    if (n_runs < 1 || (uint32_t)n_runs > geo::kEnsMaxRuns)
+      return fail(GEO_ERR_INVALID, "an ensemble holds 1 to 512 runs");
+    plasma = ps[0]->plasma;
+    magnetized = ps[0]->magnetized;
+    for (Plan<T, C>* p : ps) {
+      if (!p->bound) return fail(GEO_ERR_INVALID, "every run must have bound state");
+      if (p->plasma != plasma || p->magnetized != magnetized)
+        return fail(GEO_ERR_INVALID, "runs must all have Drude, magnetised or no plasma alike");
+      if (p->dyn_scaling)
+        return fail(GEO_ERR_INVALID, "ensembles take FP32/FP64 or statically scaled storage");
+    }
+    seen.assign(n_runs, ~0ull);
+    std::vector<geo::Dynamic*> dyns(n_runs);
+    for (int32_t r = 0; r < n_runs; ++r) dyns[r] = ps[r]->dyn;
+    GEO_HIP(hipMalloc(&d_hp, sizeof(geo::HParams<T, C>) * n_runs));
+    GEO_HIP(hipMalloc(&d_ep, sizeof(geo::EParams<T, C>) * n_runs));
+    GEO_HIP(hipMalloc(&d_dyns, sizeof(geo::Dynamic*) * n_runs));
+    GEO_HIP(hipMemcpy(d_dyns, dyns.data(), sizeof(geo::Dynamic*) * n_runs,
+                      hipMemcpyHostToDevice));
+    GEO_HIP(hipStreamCreateWithFlags(&capture_stream, hipStreamNonBlocking));
+    return configure(1, 0);
+  }
+
+  // Per-run parameter blocks with the segment geometry of the current S.
+  void fill_params() {
+    host_hp.resize(n_runs);
+    host_ep.resize(n_runs);
+    for (int32_t r = 0; r < n_runs; ++r) {
+      Plan<T, C>& p = *plans[r];
+      host_hp[r] = p.hp;
+      host_ep[r] = p.ep;
+      host_hp[r].g_edge = make_geom(p.E, p.pn, N, S);
+      host_hp[r].g_face = make_geom(p.F, p.pm, N, S);
+      host_ep[r].g_vertex = make_geom(p.V, p.pn, N, S);
+      host_ep[r].g_edge = make_geom(p.E, p.pm, N, S);
+      seen[r] = p.generation;
+    }
+  }
+
+  int upload_params(hipStream_t stream) {
+    fill_params();
+    GEO_HIP(hipMemcpyAsync(d_hp, host_hp.data(), sizeof(geo::HParams<T, C>) * n_runs,
+                           hipMemcpyHostToDevice, stream));
+    GEO_HIP(hipMemcpyAsync(d_ep, host_ep.data(), sizeof(geo::EParams<T, C>) * n_runs,
+                           hipMemcpyHostToDevice, stream));
+    GEO_HIP(hipStreamSynchronize(stream));  // the host copies are reused
+    return GEO_OK;
+  }
+
+  static std::vector<uint32_t> combine(const std::vector<std::vector<uint32_t>>& per_run,
+                                       int32_t policy) {
+    std::vector<uint32_t> out;
+    if (policy == 0) {
+      for (const auto& v : per_run) out.insert(out.end(), v.begin(), v.end());
+      return out;
+    }
+    struct Item { double key; uint32_t run; uint32_t value; };
+    std::vector<Item> items;
+    for (size_t r = 0; r < per_run.size(); ++r)
+      for (size_t i = 0; i < per_run[r].size(); ++i)
+        items.push_back({(i + 0.5) / per_run[r].size(), (uint32_t)r, per_run[r][i]});
+    std::stable_sort(items.begin(), items.end(), [](const Item& a, const Item& b) {
+      return a.key < b.key || (a.key == b.key && a.run < b.run);
+    });
+    out.reserve(items.size());
+    for (const Item& it : items) out.push_back(it.value);
+    return out;
+  }
+
+  static int encode(std::vector<uint32_t>& map, uint32_t run) {
+    for (uint32_t& m : map) {
+      const uint32_t seg = m >> 24, b = m & 0xffffffu;
+      if (b >= geo::kEnsMaxBlocks) return fail(GEO_ERR_INVALID, "run too large for an ensemble");
+      m = (seg << (geo::kEnsBlockBits + geo::kEnsRunBits)) | (run << geo::kEnsBlockBits) | b;
+    }
+    return GEO_OK;
+  }
+
+  int configure(int32_t s, int32_t pol) override {
+    if (!(s == 1 || s == 2 || s == 4)) return fail(GEO_ERR_INVALID, "strips per thread: 1, 2 or 4");
+    if (pol != 0 && pol != 1) return fail(GEO_ERR_INVALID, "block order policy: 0 or 1");
+    S = s;
+    policy = pol;
+    drop_graph();
+    fill_params();
+    std::vector<std::vector<uint32_t>> hm(n_runs), em(n_runs);
+    int rc;
+    for (int32_t r = 0; r < n_runs; ++r) {
+      const Plan<T, C>& p = *plans[r];
+      std::vector<SegSpec> hs = {{geo::kSegHEdge, geom_blocks(host_hp[r].g_edge)},
+                                 {geo::kSegHFace, geom_blocks(host_hp[r].g_face)}};
+      if (p.si_on) hs.push_back({geo::kSegSi, (int32_t)((p.E + geo::kBlock - 1) / geo::kBlock)});
+      hs.push_back({geo::kSegSample, 1});
+      hm[r] = build_block_map(hs);
+      em[r] = build_block_map({{geo::kSegEVertex, geom_blocks(host_ep[r].g_vertex)},
+                               {geo::kSegEEdge, geom_blocks(host_ep[r].g_edge)}});
+      if ((rc = encode(hm[r], (uint32_t)r)) != GEO_OK) return rc;
+      if ((rc = encode(em[r], (uint32_t)r)) != GEO_OK) return rc;
+    }
+    const std::vector<uint32_t> hmap = combine(hm, policy), emap = combine(em, policy);
+    for (void* p : {(void*)d_hmap, (void*)d_emap})
+      if (p) (void)hipFree(p);
+    GEO_HIP(hipMalloc(&d_hmap, sizeof(uint32_t) * hmap.size()));
+    GEO_HIP(hipMalloc(&d_emap, sizeof(uint32_t) * emap.size()));
+    GEO_HIP(hipMemcpy(d_hmap, hmap.data(), sizeof(uint32_t) * hmap.size(), hipMemcpyHostToDevice));
+    GEO_HIP(hipMemcpy(d_emap, emap.data(), sizeof(uint32_t) * emap.size(), hipMemcpyHostToDevice));
+    h_blocks = (int32_t)hmap.size();
+    e_blocks = (int32_t)emap.size();
+    h_per_xcd = (h_blocks + geo::kNumXcd - 1) / geo::kNumXcd;
+    e_per_xcd = (e_blocks + geo::kNumXcd - 1) / geo::kNumXcd;
+    GEO_HIP(hipMemcpy(d_hp, host_hp.data(), sizeof(geo::HParams<T, C>) * n_runs,
+                      hipMemcpyHostToDevice));
+    GEO_HIP(hipMemcpy(d_ep, host_ep.data(), sizeof(geo::EParams<T, C>) * n_runs,
+                      hipMemcpyHostToDevice));
+    return GEO_OK;
+  }
+
+  template <bool P, bool M, int SS>
+  void launch_pair(hipStream_t s) {
+    hipLaunchKernelGGL((geo::h_ensemble_kernel<T, C, P, M, SS>), dim3(h_per_xcd * geo::kNumXcd),
+                       dim3(geo::kBlock), 0, s, (const geo::HParams<T, C>*)d_hp,
+                       (const uint32_t*)d_hmap, h_blocks, h_per_xcd);
+    hipLaunchKernelGGL((geo::e_ensemble_kernel<T, C, P, M, SS>), dim3(e_per_xcd * geo::kNumXcd),
+                       dim3(geo::kBlock), 0, s, (const geo::EParams<T, C>*)d_ep,
+                       (const uint32_t*)d_emap, e_blocks, e_per_xcd);
+  }
+
+  template <int SS>
+  void launch_variant(hipStream_t s) {
+    if (plasma) {
+      if (magnetized) launch_pair<true, true, SS>(s);
+      else launch_pair<true, false, SS>(s);
+    } else {
+      launch_pair<false, false, SS>(s);
+    }
+  }
+
+  void launch_step(hipStream_t s) {
+    if (S == 1) launch_variant<1>(s);
+    else if (S == 2) launch_variant<2>(s);
+    else launch_variant<4>(s);
+  }
+
+  int ensure_graph(int32_t chunk) {
+    if (graph_exec && graph_chunk == chunk) return GEO_OK;
+    drop_graph();
+    GEO_HIP(hipStreamBeginCapture(capture_stream, hipStreamCaptureModeThreadLocal));
+    for (int32_t i = 0; i < chunk; ++i) launch_step(capture_stream);
+    hipGraph_t g = nullptr;
+    GEO_HIP(hipStreamEndCapture(capture_stream, &g));
+    graph = g;
+    GEO_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    graph_chunk = chunk;
+    return GEO_OK;
+  }
+
+  int refresh(hipStream_t stream) {
+    for (int32_t r = 0; r < n_runs; ++r)
+      if (!plans[r]->bound) return fail(GEO_ERR_INVALID, "a run has no bound state");
+    bool stale = false;
+    for (int32_t r = 0; r < n_runs; ++r) stale = stale || seen[r] != plans[r]->generation;
+    return stale ? upload_params(stream) : GEO_OK;
+  }
+
+  // currents: (n_runs, n) compute type or NULL; traces: (n_runs, n + 1, n_rx).
+  int step(const void* currents, int64_t n, int32_t chunk, void* traces,
+           hipStream_t stream) override {
+    if (n < 0) return fail(GEO_ERR_INVALID, "step count must be non-negative");
+    int rc = refresh(stream);
+    if (rc != GEO_OK) return rc;
+    int32_t n_rx = plans[0]->n_rx;
+    if (traces != nullptr) {
+      for (Plan<T, C>* p : plans)
+        if (p->n_rx != n_rx || n_rx <= 0)
+          return fail(GEO_ERR_INVALID, "trace recording needs the same receiver count in every run");
+    }
+    const int threads = 256;
+    hipLaunchKernelGGL(geo::set_dynamic_runs_kernel, dim3((n_runs + threads - 1) / threads),
+                       dim3(threads), 0, stream, (geo::Dynamic* const*)d_dyns, n_runs,
+                       static_cast<const char*>(currents), (int64_t)(n * sizeof(C)),
+                       static_cast<char*>(traces), (int64_t)((n + 1) * n_rx * sizeof(C)),
+                       traces ? n + 1 : (int64_t)0, n);
+    GEO_HIP(hipGetLastError());
+    int64_t done = 0;
+    if (chunk > 1 && n >= chunk) {
+      if ((rc = ensure_graph(chunk)) != GEO_OK) return rc;
+      for (; done + chunk <= n; done += chunk) GEO_HIP(hipGraphLaunch(graph_exec, stream));
+    }
+    for (; done < n; ++done) launch_step(stream);
+    if (traces != nullptr)
+      hipLaunchKernelGGL((geo::sample_runs_kernel<T, C>), dim3(n_runs), dim3(geo::kBlock), 0,
+                         stream, (const geo::HParams<T, C>*)d_hp);
+    GEO_HIP(hipGetLastError());
+    return GEO_OK;
+  }
+
+  // Device bytes of one run's state (fields, surface-impedance memory, plasma).
+  struct StateSpan { void* ptr; size_t bytes; };
+  std::vector<StateSpan> state_spans() const {
+    std::vector<StateSpan> v;
+    for (const Plan<T, C>* p : plans) {
+      v.push_back({(void*)p->hp.er, sizeof(T) * p->V * p->pn});
+      v.push_back({(void*)p->hp.ht, sizeof(T) * p->E * p->pn});
+      v.push_back({(void*)p->hp.et, sizeof(T) * p->E * p->pm});
+      v.push_back({(void*)p->hp.hr, sizeof(T) * p->F * p->pm});
+      if (p->si_on) v.push_back({(void*)p->hp.si_mem, sizeof(C) * p->si_terms * p->E});
+      if (p->plasma)
+        v.push_back({(void*)p->hp.pl.current, sizeof(C) * p->hp.pl.n_species *
+                                                  p->layout.plasma_components * p->F *
+                                                  p->layout.plasma_pitch});
+    }
+    return v;
+  }
+
+  // Time every (S, policy) candidate over `steps` source-free steps and keep
+  // the fastest; the state of every run is restored afterwards.
+  int tune(int32_t steps, hipStream_t stream, int32_t* best_s, int32_t* best_policy,
+           double* best_ms) override {
+    if (steps < 1) return fail(GEO_ERR_INVALID, "tuning needs at least one step");
+    int rc = refresh(stream);
+    if (rc != GEO_OK) return rc;
+    const std::vector<StateSpan> spans = state_spans();
+    size_t total = 0;
+    for (const StateSpan& sp : spans) total += round_up((int64_t)sp.bytes, 256);
+    char* save = nullptr;
+    GEO_HIP(hipMalloc(&save, total));
+    size_t off = 0;
+    for (const StateSpan& sp : spans) {
+      GEO_HIP(hipMemcpyAsync(save + off, sp.ptr, sp.bytes, hipMemcpyDeviceToDevice, stream));
+      off += round_up((int64_t)sp.bytes, 256);
+    }
+    hipEvent_t a, b;
+    GEO_HIP(hipEventCreate(&a));
+    GEO_HIP(hipEventCreate(&b));
+    double best = 1e300;
+    int32_t bs = 1, bp = 0;
+    for (int32_t pol : {0, 1}) {
+      for (int32_t s : {1, 2, 4}) {
+        if ((rc = configure(s, pol)) != GEO_OK) break;
+        if ((rc = step(nullptr, 2, 1, nullptr, stream)) != GEO_OK) break;  // warm-up
+        GEO_HIP(hipEventRecord(a, stream));
+        if ((rc = step(nullptr, steps, 1, nullptr, stream)) != GEO_OK) break;
+        GEO_HIP(hipEventRecord(b, stream));
+        GEO_HIP(hipEventSynchronize(b));
+        float ms = 0.f;
+        GEO_HIP(hipEventElapsedTime(&ms, a, b));
+        if (ms / steps < best) {
+          best = ms / steps;
+          bs = s;
+          bp = pol;
+        }
+      }
+      if (rc != GEO_OK) break;
+    }
+    off = 0;
+    for (const StateSpan& sp : spans) {
+      (void)hipMemcpyAsync(sp.ptr, save + off, sp.bytes, hipMemcpyDeviceToDevice, stream);
+      off += round_up((int64_t)sp.bytes, 256);
+    }
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(save);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc != GEO_OK) return rc;
+    if ((rc = configure(bs, bp)) != GEO_OK) return rc;
+    *best_s = bs;
+    *best_policy = bp;
+    *best_ms = best;
+    return GEO_OK;
+  }
+};
+
+template <typename T, typename C>
+int make_ensemble(PlanBase* const* bases, int32_t n, std::unique_ptr<EnsembleBase>* out) {
+  std::vector<Plan<T, C>*> ps(n);
+  for (int32_t i = 0; i < n; ++i) {
+    ps[i] = dynamic_cast<Plan<T, C>*>(bases[i]);
+    if (!ps[i]) return fail(GEO_ERR_INVALID, "runs of an ensemble must share the storage type");
+  }
+  auto e = std::make_unique<Ensemble<T, C>>();
+  const int rc = e->create(ps);
+  if (rc != GEO_OK) return rc;
+  *out = std::move(e);
+  return GEO_OK;
+}
+
 int check_device() {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count < 1)
@@ -1259,6 +1614,10 @@ int check_device() {
 
 struct GeoPlan {
   std::unique_ptr<PlanBase> impl;
+};
+
+struct GeoEnsemble {
+  std::unique_ptr<EnsembleBase> impl;
 };
 
 extern "C" {
@@ -1358,6 +1717,50 @@ int geo_set_storage_scales(GeoPlan* plan, const double* in4, void* stream) {
 }
 
 void geo_plan_destroy(GeoPlan* plan) { delete plan; }
+
+int geo_ensemble_create(GeoPlan* const* plans, int32_t n_plans, GeoEnsemble** out) {
+  if (!plans || !out || n_plans < 1) return fail(GEO_ERR_INVALID, "null argument");
+  *out = nullptr;
+  std::vector<PlanBase*> bases(n_plans);
+  for (int32_t i = 0; i < n_plans; ++i) {
+    if (!plans[i]) return fail(GEO_ERR_INVALID, "null plan");
+    bases[i] = plans[i]->impl.get();
+  }
+  std::unique_ptr<EnsembleBase> impl;
+  int rc = GEO_ERR_INVALID;
+  if (dynamic_cast<Plan<float, float>*>(bases[0]))
+    rc = make_ensemble<float, float>(bases.data(), n_plans, &impl);
+  else if (dynamic_cast<Plan<double, double>*>(bases[0]))
+    rc = make_ensemble<double, double>(bases.data(), n_plans, &impl);
+  else if (dynamic_cast<Plan<__half, float>*>(bases[0]))
+    rc = make_ensemble<__half, float>(bases.data(), n_plans, &impl);
+  else if (dynamic_cast<Plan<__hip_bfloat16, float>*>(bases[0]))
+    rc = make_ensemble<__hip_bfloat16, float>(bases.data(), n_plans, &impl);
+  if (rc != GEO_OK) return rc;
+  *out = new GeoEnsemble{std::move(impl)};
+  return GEO_OK;
+}
+
+int geo_ensemble_configure(GeoEnsemble* ens, int32_t strips_per_thread, int32_t block_order) {
+  if (!ens) return fail(GEO_ERR_INVALID, "null argument");
+  return ens->impl->configure(strips_per_thread, block_order);
+}
+
+int geo_ensemble_tune(GeoEnsemble* ens, int32_t steps, void* stream, int32_t* strips_per_thread,
+                      int32_t* block_order, double* ms_per_step) {
+  if (!ens || !strips_per_thread || !block_order || !ms_per_step)
+    return fail(GEO_ERR_INVALID, "null argument");
+  return ens->impl->tune(steps, static_cast<hipStream_t>(stream), strips_per_thread, block_order,
+                         ms_per_step);
+}
+
+int geo_ensemble_step(GeoEnsemble* ens, const void* currents, int64_t n_steps, int32_t chunk,
+                      void* traces, void* stream) {
+  if (!ens) return fail(GEO_ERR_INVALID, "null argument");
+  return ens->impl->step(currents, n_steps, chunk, traces, static_cast<hipStream_t>(stream));
+}
+
+void geo_ensemble_destroy(GeoEnsemble* ens) { delete ens; }
 
 int geo_stream_probe(void* dst, const void* src, int64_t bytes, int32_t mode, int32_t repeats,
                      void* stream, double* out_ms) {

@@ -44,9 +44,10 @@ enum Segment : uint32_t {
 // (exact for tid < 256 and strips <= 256).
 struct SegGeom {
   int32_t n_rows;
-  int32_t strips;         // 16-byte strips per row
+  int32_t strips;         // thread columns per row: ceil(row_strips / S)
   int32_t rows_per_block;
   uint32_t magic;         // ceil(65536 / strips)
+  int32_t row_strips;     // 16-byte strips per row
 };
 
 struct Dynamic {
@@ -561,28 +562,48 @@ __device__ __forceinline__ void sample_block(const HParams<T, C>& p, const Scale
   if (threadIdx.x == 0) p.dyn->counter = r + 1;
 }
 
-template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
-__global__ void __launch_bounds__(kBlock) h_step_kernel(const HParams<T, C> p) {
-  const int32_t L = logical_block(p.per_xcd);
-  if (L >= p.n_blocks) return;
-  const Scales<C> sc = load_scales<T, C>(p.dyn);
-  const uint32_t m = p.block_map[L];
-  const uint32_t seg = m >> 24;
-  const int32_t b = (int32_t)(m & 0xffffffu);
+// One block of the H kernel.  With S strips per thread (a column block of
+// S x 16 bytes), thread column c of a row handles strips c, c + cols, ...,
+// so every load instruction of a wave stays coalesced along the row.
+template <typename T, typename C, bool PLASMA, bool MAGNETIZED, int S>
+__device__ __forceinline__ void h_dispatch(const HParams<T, C>& p, const Scales<C>& sc,
+                                           uint32_t seg, int32_t b) {
   if (seg == kSegHEdge) {
-    int32_t row, strip;
-    row_and_strip(p.g_edge, b, &row, &strip);
-    if (row < p.g_edge.n_rows) h_edge_strip(p, sc, row, strip);
+    int32_t row, col;
+    row_and_strip(p.g_edge, b, &row, &col);
+    if (row < p.g_edge.n_rows) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int32_t strip = col + s * p.g_edge.strips;
+        if (S == 1 || strip < p.g_edge.row_strips) h_edge_strip(p, sc, row, strip);
+      }
+    }
   } else if (seg == kSegHFace) {
-    int32_t row, strip;
-    row_and_strip(p.g_face, b, &row, &strip);
-    if (row < p.g_face.n_rows) h_face_strip<T, C, PLASMA, MAGNETIZED>(p, sc, row, strip);
+    int32_t row, col;
+    row_and_strip(p.g_face, b, &row, &col);
+    if (row < p.g_face.n_rows) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int32_t strip = col + s * p.g_face.strips;
+        if (S == 1 || strip < p.g_face.row_strips)
+          h_face_strip<T, C, PLASMA, MAGNETIZED>(p, sc, row, strip);
+      }
+    }
   } else if (seg == kSegSi) {
     const int32_t e = b * kBlock + (int32_t)threadIdx.x;
     if (e < p.n_edges) h_si_edge(p, sc, e);
   } else {
     sample_block(p, sc);
   }
+}
+
+template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
+__global__ void __launch_bounds__(kBlock) h_step_kernel(const HParams<T, C> p) {
+  const int32_t L = logical_block(p.per_xcd);
+  if (L >= p.n_blocks) return;
+  const Scales<C> sc = load_scales<T, C>(p.dyn);
+  const uint32_t m = p.block_map[L];
+  h_dispatch<T, C, PLASMA, MAGNETIZED, 1>(p, sc, m >> 24, (int32_t)(m & 0xffffffu));
 }
 
 // ---------------------------------------------------------------------------
@@ -800,22 +821,104 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
   store_strip(erow, et, sc.et);
 }
 
+template <typename T, typename C, bool PLASMA, bool MAGNETIZED, int S>
+__device__ __forceinline__ void e_dispatch(const EParams<T, C>& p, const Scales<C>& sc,
+                                           uint32_t seg, int32_t b) {
+  int32_t row, col;
+  if (seg == kSegEVertex) {
+    row_and_strip(p.g_vertex, b, &row, &col);
+    if (row < p.g_vertex.n_rows) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int32_t strip = col + s * p.g_vertex.strips;
+        if (S == 1 || strip < p.g_vertex.row_strips)
+          e_vertex_strip<T, C, PLASMA, MAGNETIZED>(p, sc, row, strip);
+      }
+    }
+  } else {
+    row_and_strip(p.g_edge, b, &row, &col);
+    if (row < p.g_edge.n_rows) {
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+        const int32_t strip = col + s * p.g_edge.strips;
+        if (S == 1 || strip < p.g_edge.row_strips)
+          e_edge_strip<T, C, PLASMA, MAGNETIZED>(p, sc, row, strip);
+      }
+    }
+  }
+}
+
 template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
 __global__ void __launch_bounds__(kBlock) e_step_kernel(const EParams<T, C> p) {
   const int32_t L = logical_block(p.per_xcd);
   if (L >= p.n_blocks) return;
   const Scales<C> sc = load_scales<T, C>(p.dyn);
   const uint32_t m = p.block_map[L];
-  const uint32_t seg = m >> 24;
-  const int32_t b = (int32_t)(m & 0xffffffu);
-  int32_t row, strip;
-  if (seg == kSegEVertex) {
-    row_and_strip(p.g_vertex, b, &row, &strip);
-    if (row < p.g_vertex.n_rows) e_vertex_strip<T, C, PLASMA, MAGNETIZED>(p, sc, row, strip);
-  } else {
-    row_and_strip(p.g_edge, b, &row, &strip);
-    if (row < p.g_edge.n_rows) e_edge_strip<T, C, PLASMA, MAGNETIZED>(p, sc, row, strip);
-  }
+  e_dispatch<T, C, PLASMA, MAGNETIZED, 1>(p, sc, m >> 24, (int32_t)(m & 0xffffffu));
+}
+
+// ---------------------------------------------------------------------------
+// Ensembles: independent runs (own radial grid, coefficients, plasma, ground,
+// source and receivers; shared mesh) advanced by the same two launches per
+// step.  Each run's parameter block lives in a device array; a map entry
+// names the segment, the run and the block within that run's segment.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kEnsBlockBits = 20;
+constexpr uint32_t kEnsRunBits = 9;
+constexpr uint32_t kEnsMaxRuns = 1u << kEnsRunBits;
+constexpr uint32_t kEnsMaxBlocks = 1u << kEnsBlockBits;
+
+__device__ __forceinline__ void ens_decode(uint32_t m, uint32_t* seg, int32_t* run, int32_t* b) {
+  *seg = m >> (kEnsBlockBits + kEnsRunBits);
+  *run = (int32_t)((m >> kEnsBlockBits) & (kEnsMaxRuns - 1));
+  *b = (int32_t)(m & (kEnsMaxBlocks - 1));
+}
+
+template <typename T, typename C, bool PLASMA, bool MAGNETIZED, int S>
+__global__ void __launch_bounds__(kBlock) h_ensemble_kernel(
+    const HParams<T, C>* __restrict__ runs, const uint32_t* __restrict__ map, int32_t n_blocks,
+    int32_t per_xcd) {
+  const int32_t L = logical_block(per_xcd);
+  if (L >= n_blocks) return;
+  uint32_t seg;
+  int32_t run, b;
+  ens_decode(map[L], &seg, &run, &b);
+  const HParams<T, C>& p = runs[run];
+  h_dispatch<T, C, PLASMA, MAGNETIZED, S>(p, load_scales<T, C>(p.dyn), seg, b);
+}
+
+template <typename T, typename C, bool PLASMA, bool MAGNETIZED, int S>
+__global__ void __launch_bounds__(kBlock) e_ensemble_kernel(
+    const EParams<T, C>* __restrict__ runs, const uint32_t* __restrict__ map, int32_t n_blocks,
+    int32_t per_xcd) {
+  const int32_t L = logical_block(per_xcd);
+  if (L >= n_blocks) return;
+  uint32_t seg;
+  int32_t run, b;
+  ens_decode(map[L], &seg, &run, &b);
+  const EParams<T, C>& p = runs[run];
+  e_dispatch<T, C, PLASMA, MAGNETIZED, S>(p, load_scales<T, C>(p.dyn), seg, b);
+}
+
+// Final trace row of every run (block r samples run r and advances its counter).
+template <typename T, typename C>
+__global__ void __launch_bounds__(kBlock) sample_runs_kernel(const HParams<T, C>* __restrict__ runs) {
+  const HParams<T, C>& p = runs[blockIdx.x];
+  sample_block(p, load_scales<T, C>(p.dyn));
+}
+
+__global__ void set_dynamic_runs_kernel(Dynamic* const* __restrict__ dyns, int32_t n,
+                                        const char* currents, int64_t currents_stride,
+                                        char* trace, int64_t trace_stride, int64_t rows,
+                                        int64_t n_currents) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  Dynamic* d = dyns[i];
+  d->counter = 0;
+  d->currents = currents != nullptr ? currents + i * currents_stride : nullptr;
+  d->n_currents = currents != nullptr ? n_currents : 0;
+  d->trace = trace != nullptr ? trace + i * trace_stride : nullptr;
+  d->trace_rows = trace != nullptr ? rows : 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -640,11 +640,30 @@ class GeodesicFDTD:
         samples are written by the step kernels into a device trace buffer;
         one device-to-host copy happens at the end.
         """
+        steps = self._validated_count(steps, "step count", minimum=0)
+        self._validated_count(synchronize_every, "synchronize_every", minimum=1)
+        n_rx = self._set_receivers(vertex_indices, radial_layers, weights)
+        torch = self.backend.torch
+        traces = torch.zeros((steps + 1, n_rx), dtype=self.backend.compute_dtype,
+                             device=self.backend.torch_device)
+        if n_rx == 0:
+            self.step(steps)
+            return self.to_numpy(traces)
+        currents = self._source_currents(steps)
+        chunk = self.compile_chunk_size if self.compiled else 1
+        _native.check(self._lib.geo_step(
+            self._plan, None if currents is None else currents.data_ptr(), steps, chunk,
+            traces.data_ptr(), self._stream()), self._lib)
+        self.steps += steps
+        self.time_s = self.steps * self.time_step_s
+        self.backend.synchronize()
+        return self.to_numpy(traces)
+
+    def _set_receivers(self, vertex_indices, radial_layers, weights) -> int:
+        """Validate receivers (solver.py:1097-1120) and upload them to the plan."""
         vertices = self._validated_index_array(vertex_indices, "vertex_indices")
         layers = self._validated_index_array(radial_layers, "radial_layers")
         w = np.asarray(weights, dtype=np.float64)
-        steps = self._validated_count(steps, "step count", minimum=0)
-        self._validated_count(synchronize_every, "synchronize_every", minimum=1)
         if vertices.ndim != 2 or w.shape != vertices.shape:
             raise ValueError("vertex_indices and weights must have matching 2-D shapes")
         if layers.shape != (vertices.shape[0],):
@@ -661,31 +680,19 @@ class GeodesicFDTD:
         width = vertices.shape[1]
         if width > 3:
             raise ValueError("each observation may combine at most three vertices")
+        if n_rx == 0:
+            return 0
         v3 = np.zeros((n_rx, 3), dtype=np.int64)
         w3 = np.zeros((n_rx, 3), dtype=np.float64)
         v3[:, :width] = vertices
         v3[:, width:] = vertices[:, :1]
         w3[:, :width] = w
-        torch = self.backend.torch
-        traces = torch.zeros((steps + 1, n_rx), dtype=self.backend.compute_dtype,
-                             device=self.backend.torch_device)
-        if n_rx == 0:
-            self.step(steps)
-            return self.to_numpy(traces)
         v3c, l3c, w3c = (np.ascontiguousarray(v3), np.ascontiguousarray(layers),
                          np.ascontiguousarray(w3))
         _native.check(self._lib.geo_set_receivers(self._plan, _native.i64_ptr(v3c),
                                                   _native.i64_ptr(l3c), _native.f64_ptr(w3c),
                                                   n_rx), self._lib)
-        currents = self._source_currents(steps)
-        chunk = self.compile_chunk_size if self.compiled else 1
-        _native.check(self._lib.geo_step(
-            self._plan, None if currents is None else currents.data_ptr(), steps, chunk,
-            traces.data_ptr(), self._stream()), self._lib)
-        self.steps += steps
-        self.time_s = self.steps * self.time_step_s
-        self.backend.synchronize()
-        return self.to_numpy(traces)
+        return n_rx
 
     def record_h_observations(self, face_indices, face_radial_layers, face_weights,
                               edge_indices, edge_radial_layers, edge_weights, steps: int, *,

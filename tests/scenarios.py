@@ -145,6 +145,39 @@ def scenario(name: str, **overrides) -> dict:
     return base
 
 
+C2_SEED = 20260902
+
+
+def c2_runs(n_runs: int = 64, seed: int = C2_SEED, subdivision: int = 5,
+            radial_choices=(32, 40, 48), n_receivers: int = 16) -> list[dict]:
+    """Seeded scenarios of the batched ensemble: per run R in radial_choices,
+    h' ~ U(68, 80) km, beta ~ U(0.25, 0.55) /km (electron Drude profile as c1),
+    ground sigma ~ logU(1e-4, 1e-1) S/m (SI boundary), source lat/lon uniform
+    on the sphere, carrier ~ U(10, 40) Hz, FP32, receivers uniform on the
+    sphere; the c0 conductivity profile and r_E = 6370 km throughout."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_runs):
+        R = int(rng.choice(radial_choices))
+        h_prime = float(rng.uniform(68.0, 80.0))
+        beta = float(rng.uniform(0.25, 0.55))
+        sigma_g = float(10.0 ** rng.uniform(-4.0, -1.0))
+        lat = float(np.rad2deg(np.arcsin(rng.uniform(-1.0, 1.0))))
+        lon = float(rng.uniform(-180.0, 180.0))
+        carrier = float(rng.uniform(10.0, 40.0))
+        rx_lat = np.rad2deg(np.arcsin(rng.uniform(-1.0, 1.0, n_receivers)))
+        rx_lon = rng.uniform(-180.0, 180.0, n_receivers)
+        sc = scenario("c0", subdivision=subdivision, radial_cells=R, dtype="float32",
+                      surface_sigma=sigma_g, receivers=str(n_receivers),
+                      plasma={"h_prime_km": h_prime, "beta_per_km": beta},
+                      source={"latitude_deg": lat, "longitude_deg": lon, "altitude_m": 0.0,
+                              "carrier_frequency_hz": carrier})
+        sc["name"] = f"c2_run{i}"
+        sc["receiver_points"] = list(zip(rx_lat.tolist(), rx_lon.tolist()))
+        out.append(sc)
+    return out
+
+
 def _resolve_time(value, dt):
     if isinstance(value, str) and value.endswith("dt"):
         return float(value[:-2]) * dt
@@ -154,6 +187,8 @@ def _resolve_time(value, dt):
 def receiver_points(sc: dict) -> list[tuple[float, float]]:
     from ionosphere_fdtd_amd.sources import destination_point
 
+    if sc.get("receiver_points") is not None:
+        return list(sc["receiver_points"])
     if sc["receivers"] == "c0":
         return [destination_point(SOURCE_LAT, SOURCE_LON, 90.0, d * 1e3, sc["earth_radius_m"])
                 for d in (1000.0, 5000.0, 10000.0, 20000.0)]
@@ -202,7 +237,7 @@ def _plasma_arrays(sc, mesh, midpoint_altitudes):
 
 
 def build_sim(sc: dict, *, compile_step=False, compile_chunk_size=8, dtype=None,
-              storage_scales=None, storage_scaling="auto"):
+              storage_scales=None, storage_scaling="auto", mesh=None):
     import ionosphere_fdtd_amd as g
 
     cfg = g.SimulationConfig(
@@ -215,7 +250,8 @@ def build_sim(sc: dict, *, compile_step=False, compile_chunk_size=8, dtype=None,
         radial_grid_policy="balanced-2to1" if sc["radial_altitudes_m"] else "smooth")
     material = VacuumMaterial() if sc["vacuum"] else g.EarthIonosphereMaterial(**sc["material"])
     surface = g.ConductiveHalfSpaceSurface(sc["surface_sigma"]) if sc["surface_sigma"] else None
-    mesh = g.build_geodesic_mesh(sc["subdivision"])
+    if mesh is None:
+        mesh = g.build_geodesic_mesh(sc["subdivision"])
     plasma = None
     if sc["plasma"]:
         alt = _midpoint_altitudes(sc)
