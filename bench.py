@@ -168,6 +168,78 @@ def precision_sweep(args) -> dict:
     return out
 
 
+def ensemble_bench(args) -> dict:
+    """Batched ensemble: 64 seeded runs on the s5 mesh (R in {32, 40, 48},
+    own h', beta, ground sigma, source, carrier, 16 receivers each), FP32,
+    advanced together in one H and one E launch per step (HIP graphs of
+    --chunk steps) with the autotuned column-block shape.  The same runs
+    stepped one after another (each with its own graphs) are timed beside it."""
+    import numpy as np
+    import torch
+
+    import scenarios as S
+
+    import ionosphere_fdtd_amd as g
+
+    t0 = time.perf_counter()
+    scs = S.c2_runs(n_runs=args.runs)
+    mesh = g.build_geodesic_mesh(scs[0]["subdivision"])
+    runs = [S.build_sim(sc, mesh=mesh, compile_step=True, compile_chunk_size=args.chunk)
+            for sc in scs]
+    rx = [S.receiver_tables(mesh, r.altitudes_m, S.receiver_points(sc)) for r, sc in zip(runs, scs)]
+    print(f"ensemble setup {time.perf_counter() - t0:.1f} s, {len(runs)} runs", file=sys.stderr,
+          flush=True)
+    ens = g.EnsembleFDTD(runs, compile_chunk_size=args.chunk, autotune=False)
+    shape = ens.tune(10)
+    print(f"autotuned shape {shape}", file=sys.stderr, flush=True)
+    ens.record_er_observations(rx, args.warmup)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    traces = ens.record_er_observations(rx, args.steps)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    assert np.all(np.isfinite(traces))
+    cells = ens.cells
+    step_bytes = sum(algorithmic_bytes(r)["h"] + algorithmic_bytes(r)["e"] for r in runs)
+    # the same runs, one after another
+    solo_steps = max(args.steps // 10, args.chunk)
+    for r, x in zip(runs, rx):
+        r.record_er_observations(*x, args.chunk)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    for r, x in zip(runs, rx):
+        r.record_er_observations(*x, solo_steps)
+    torch.cuda.synchronize()
+    solo = time.perf_counter() - t2
+    n_traces = len(runs) * traces.shape[2]
+    return {
+        "metric": "ensemble Mcell-updates/s (64 uneven runs, one launch per half-step)",
+        "value": round(args.steps * cells / elapsed / 1e6, 1),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(runs)} runs, s5 mesh, R in {{32,40,48}} (seeded), Drude "
+                               "plasma h'~U(68,80) km beta~U(0.25,0.55)/km, SI ground "
+                               "sigma~logU(1e-4,1e-1), carrier~U(10,40) Hz, 16 receivers each",
+                   "cells": cells,
+                   "radial_cells": [len(r.radial_midpoints_m) for r in runs],
+                   "graph_chunk": args.chunk, "column_block": shape},
+        "trace_queries_per_s": round(n_traces / elapsed, 2),
+        "traces_per_ensemble_run": n_traces,
+        "roofline": {"bound": "hbm", "kernel": "h_ensemble_kernel+e_ensemble_kernel",
+                     "achieved": round(step_bytes / (elapsed / args.steps) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(step_bytes / (elapsed / args.steps) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "algorithmic_bytes_per_step": step_bytes},
+        "runs_one_after_another": {
+            "ms_per_step": round(solo * 1e3 / solo_steps, 5),
+            "value": round(solo_steps * cells / solo / 1e6, 1),
+            "ensemble_speedup": round((solo / solo_steps) / (elapsed / args.steps), 2)},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,7 +249,8 @@ def main() -> int:
     ap.add_argument("--kernel-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="c1",
-                    help="c1 (default), c1_noplasma, c1_pec, c3, vacuum_s7_r64, vacuum_s7_r80")
+                    help="c1 (default), c1_noplasma, c1_pec, c2 (64-run ensemble), c3, "
+                         "vacuum_s7_r64, vacuum_s7_r80")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="also time a 1 GiB device copy (read + write GB/s)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -186,6 +259,7 @@ def main() -> int:
                     help="run the FP64/FP32/FP16/BF16 storage sweep on --workload (c3: s8 x 96, "
                          "refined 60-90 km band) for --steps steps instead of the throughput "
                          "bench")
+    ap.add_argument("--runs", type=int, default=64, help="ensemble runs (--workload c2)")
     ap.add_argument("--sweep-dtypes", default="float32,float16,bfloat16",
                     help="storage types compared with the FP64 run in --precision-sweep")
     args = ap.parse_args()
@@ -199,6 +273,9 @@ def main() -> int:
     import scenarios as S
 
     torch.cuda.set_device(0)
+    if args.workload == "c2":
+        print(json.dumps(ensemble_bench(args)))
+        return 0
     if args.precision_sweep:
         sweep = precision_sweep(args)
         print(json.dumps({"metric": "storage precision sweep: error vs FP64 GPU run",
