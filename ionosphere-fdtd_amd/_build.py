@@ -8,8 +8,9 @@ from pathlib import Path
 
 PACKAGE_DIR = Path(__file__).resolve().parent
 REPO_ROOT = PACKAGE_DIR.parent
-SOURCES = [PACKAGE_DIR / "csrc" / "geofdtd.hip"]
-HEADERS = [REPO_ROOT / "include" / "geofdtd.h", PACKAGE_DIR / "csrc" / "geofdtd_kernels.h"]
+SOURCES = [PACKAGE_DIR / "csrc" / "geofdtd.hip", PACKAGE_DIR / "csrc" / "surrogate.hip"]
+HEADERS = [REPO_ROOT / "include" / "geofdtd.h", REPO_ROOT / "include" / "geosurrogate.h",
+           PACKAGE_DIR / "csrc" / "geofdtd_kernels.h", PACKAGE_DIR / "csrc" / "geofdtd_internal.h"]
 OUTPUT = PACKAGE_DIR / "libgeofdtd.so"
 
 HIPCC_FLAGS = [

@@ -100,6 +100,9 @@ EXPORTED_SYMBOLS = (
     "geo_last_error", "geo_version", "geo_stream_probe", "geo_get_storage_scales",
     "geo_set_storage_scales", "geo_ensemble_create", "geo_ensemble_configure",
     "geo_ensemble_tune", "geo_ensemble_step", "geo_ensemble_destroy",
+    # include/geosurrogate.h
+    "geo_gemm_bf16", "geo_f32_to_bf16", "geo_silu_forward", "geo_silu_backward",
+    "geo_column_sums", "geo_im2col_up", "geo_col2im_up", "geo_adam_step", "geo_trace_loss",
 )
 
 _lib = None
@@ -137,6 +140,17 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.geo_ensemble_step.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp]
     lib.geo_ensemble_destroy.argtypes = [vp]
     lib.geo_ensemble_destroy.restype = None
+    i64, i32, f32 = C.c_int64, C.c_int32, C.c_float
+    lib.geo_gemm_bf16.argtypes = [i64, i64, i64, vp, i64, i64, vp, i64, i64, vp, vp, vp, i64,
+                                  i32, vp]
+    lib.geo_f32_to_bf16.argtypes = [i64, vp, vp, vp]
+    lib.geo_silu_forward.argtypes = [i64, vp, vp, vp]
+    lib.geo_silu_backward.argtypes = [i64, vp, vp, vp, vp, vp]
+    lib.geo_column_sums.argtypes = [i64, i64, vp, i64, vp, vp]
+    lib.geo_im2col_up.argtypes = [i64, i64, i64, i32, i32, vp, vp, vp]
+    lib.geo_col2im_up.argtypes = [i64, i64, i64, i32, i32, vp, vp, vp]
+    lib.geo_adam_step.argtypes = [i64, vp, vp, vp, vp, vp, f32, f32, f32, f32, i64, vp]
+    lib.geo_trace_loss.argtypes = [i64, i64, vp, vp, vp, i64, vp, f32, vp, vp, vp, vp, vp]
     lib.geo_last_error.restype = C.c_char_p
     lib.geo_last_error.argtypes = []
     lib.geo_version.restype = C.c_int

@@ -6,6 +6,7 @@
 // owns them until geo_plan_destroy.  Field state stays in caller buffers.
 #include "geofdtd.h"
 #include "geofdtd_kernels.h"
+#include "geofdtd_internal.h"
 
 #include <algorithm>
 #include <cfloat>
@@ -1611,6 +1612,8 @@ int check_device() {
 }
 
 }  // namespace
+
+int geo_internal_fail(int code, const char* msg) { return fail(code, msg); }
 
 struct GeoPlan {
   std::unique_ptr<PlanBase> impl;

@@ -1,0 +1,400 @@
+// Kernels of the trace surrogate (include/geosurrogate.h): a BF16 MFMA GEMM
+// with FP32 accumulation for every matrix product of the network's forward
+// and backward passes, and the elementwise / reduction / layout kernels
+// around it.  gfx950 only.
+#include "geofdtd.h"
+#include "geosurrogate.h"
+#include "geofdtd_internal.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <string>
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+#define SG_HIP(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      return geo_internal_fail(GEO_ERR_HIP,                                                 \
+                               (std::string(#expr) + " failed: " + hipGetErrorString(_e)).c_str()); \
+  } while (0)
+
+// Round to nearest even; NaN stays a quiet NaN.
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7fffffffu) > 0x7f800000u) return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + __expf(-x)); }
+
+// ---------------------------------------------------------------------------
+// GEMM.  Block tile 64 (M) x 64 (N), K step 32, 256 threads = 4 waves; wave w
+// owns the 32 x 32 quadrant (w / 2, w % 2) as 2 x 2 tiles of
+// v_mfma_f32_16x16x32_bf16.  Operand layout of that instruction: lane l holds
+// A[l % 16][8 (l / 16) + e] and B[8 (l / 16) + e][l % 16], e = 0..7, and
+// returns D[4 (l / 16) + r][l % 16], r = 0..3.  LDS holds A as [m][k] and B
+// transposed as [n][k], so every lane's operand is one 16-byte LDS read; rows
+// are padded by 8 values against bank conflicts.
+// ---------------------------------------------------------------------------
+constexpr int kTM = 64, kTN = 64, kTK = 32, kLdsRow = kTK + 8;
+
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(
+    int64_t M, int64_t N, int64_t K, const uint16_t* __restrict__ A, int64_t a_rs, int64_t a_cs,
+    const uint16_t* __restrict__ B, int64_t b_rs, int64_t b_cs, const float* __restrict__ bias,
+    float* __restrict__ c32, uint16_t* __restrict__ c16, int64_t ldc, int32_t accumulate) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[kTM * kLdsRow];
+  __shared__ __attribute__((aligned(16))) uint16_t Bt[kTN * kLdsRow];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int64_t m0 = (int64_t)blockIdx.y * kTM, n0 = (int64_t)blockIdx.x * kTN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  // loader coordinates: tile row (0..63) and 8 consecutive k
+  const int lr = t >> 2, lk = (t & 3) * 8;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  for (int64_t k0 = 0; k0 < K; k0 += kTK) {
+    {
+      const int64_t m = m0 + lr;
+      uint16_t v[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int64_t k = k0 + lk + e;
+        v[e] = (m < M && k < K) ? A[m * a_rs + k * a_cs] : (uint16_t)0;
+      }
+      uint4 packed;
+      packed.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+      packed.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+      packed.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
+      packed.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
+      *reinterpret_cast<uint4*>(&As[lr * kLdsRow + lk]) = packed;
+    }
+    {
+      const int64_t n = n0 + lr;
+      uint16_t v[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int64_t k = k0 + lk + e;
+        v[e] = (n < N && k < K) ? B[k * b_rs + n * b_cs] : (uint16_t)0;
+      }
+      uint4 packed;
+      packed.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+      packed.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+      packed.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
+      packed.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
+      *reinterpret_cast<uint4*>(&Bt[lr * kLdsRow + lk]) = packed;
+    }
+    __syncthreads();
+    bf16x8 a[2], b[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const uint4 x = *reinterpret_cast<const uint4*>(&As[(wm + i * 16 + r16) * kLdsRow + kq]);
+      a[i] = __builtin_bit_cast(bf16x8, x);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const uint4 x = *reinterpret_cast<const uint4*>(&Bt[(wn + j * 16 + r16) * kLdsRow + kq]);
+      b[j] = __builtin_bit_cast(bf16x8, x);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
+        const int64_t n = n0 + wn + j * 16 + r16;
+        if (m >= M || n >= N) continue;
+        float v = acc[i][j][r];
+        if (bias != nullptr) v += bias[n];
+        if (c32 != nullptr) {
+          float* p = c32 + m * ldc + n;
+          *p = accumulate ? *p + v : v;
+        }
+        if (c16 != nullptr) c16[m * ldc + n] = f2bf(v);
+      }
+}
+
+int64_t grid_for(int64_t n, int threads = 256, int64_t cap = 65536) {
+  const int64_t g = (n + threads - 1) / threads;
+  return g < 1 ? 1 : (g > cap ? cap : g);
+}
+
+__global__ void f32_to_bf16_kernel(int64_t n, const float* __restrict__ x,
+                                   uint16_t* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = f2bf(x[i]);
+}
+
+__global__ void silu_forward_kernel(int64_t n, const float* __restrict__ pre,
+                                    uint16_t* __restrict__ out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float x = pre[i];
+    out[i] = f2bf(x * sigmoid(x));
+  }
+}
+
+__global__ void silu_backward_kernel(int64_t n, const float* __restrict__ pre,
+                                     const float* __restrict__ dout, float* __restrict__ dpre,
+                                     uint16_t* __restrict__ dpre16) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float x = pre[i], s = sigmoid(x);
+    const float d = dout[i] * s * (1.0f + x * (1.0f - s));
+    dpre[i] = d;
+    if (dpre16 != nullptr) dpre16[i] = f2bf(d);
+  }
+}
+
+// One block per group of 64 columns; thread (row lane r, column c) sums rows
+// r, r + 4, ... and the 4 row lanes are combined in a fixed order.
+__global__ void __launch_bounds__(256) column_sums_kernel(int64_t M, int64_t N,
+                                                          const float* __restrict__ x, int64_t ld,
+                                                          float* __restrict__ out) {
+  __shared__ float part[4][64];
+  const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
+  const int64_t col = (int64_t)blockIdx.x * 64 + c;
+  float s = 0.f;
+  if (col < N)
+    for (int64_t m = r; m < M; m += 4) s += x[m * ld + col];
+  part[r][c] = s;
+  __syncthreads();
+  if (r == 0 && col < N) out[col] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+}
+
+__global__ void im2col_up_kernel(int64_t batch, int64_t length, int64_t channels, int32_t up,
+                                 int32_t ksize, const uint16_t* __restrict__ x,
+                                 uint16_t* __restrict__ cols) {
+  const int64_t lo = length * up, kc = (int64_t)ksize * channels;
+  const int64_t total = batch * lo * kc;
+  const int32_t half = ksize / 2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t row = i / kc, col = i - row * kc;
+    const int64_t b = row / lo, l = row - b * lo;
+    const int64_t j = col / channels, c = col - j * channels;
+    const int64_t u = l + j - half;
+    cols[i] = (u >= 0 && u < lo) ? x[(b * length + u / up) * channels + c] : (uint16_t)0;
+  }
+}
+
+__global__ void col2im_up_kernel(int64_t batch, int64_t length, int64_t channels, int32_t up,
+                                 int32_t ksize, const float* __restrict__ dcols,
+                                 float* __restrict__ dx) {
+  const int64_t lo = length * up, kc = (int64_t)ksize * channels;
+  const int64_t total = batch * length * channels;
+  const int32_t half = ksize / 2;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t bl = i / channels, c = i - bl * channels;
+    const int64_t b = bl / length, lp = bl - b * length;
+    float s = 0.f;
+    for (int32_t q = 0; q < up; ++q) {
+      const int64_t u = lp * up + q;  // upsampled position filled from x[b, lp, c]
+      for (int32_t j = 0; j < ksize; ++j) {
+        const int64_t l = u - j + half;  // output row whose tap j reads u
+        if (l >= 0 && l < lo) s += dcols[(b * lo + l) * kc + j * channels + c];
+      }
+    }
+    dx[i] = s;
+  }
+}
+
+__global__ void adam_kernel(int64_t n, float* __restrict__ p, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v, uint16_t* __restrict__ p16,
+                            float lr, float b1, float b2, float eps, float c1, float c2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    p[i] = pi;
+    if (p16 != nullptr) p16[i] = f2bf(pi);
+  }
+}
+
+constexpr int kLossBlocks = 1024;
+
+__global__ void __launch_bounds__(256) trace_loss_kernel(
+    int64_t batch, int64_t length, const float* __restrict__ s, const float* __restrict__ st,
+    const float* __restrict__ a, int64_t a_ld, const float* __restrict__ at, float w,
+    float* __restrict__ ds, float* __restrict__ da, float* __restrict__ partial) {
+  __shared__ float red[256];
+  const int64_t total = batch * length;
+  const float inv_shape = 1.0f / (float)total;
+  const float inv_batch = 1.0f / (float)batch;
+  float acc = 0.f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float d = s[i] - st[i];
+    acc += d * d * inv_shape;
+    ds[i] = 2.f * d * inv_shape;
+  }
+  for (int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; b < batch;
+       b += (int64_t)gridDim.x * blockDim.x) {
+    const float d = a[b * a_ld] - at[b];
+    acc += w * d * d * inv_batch;
+    da[b * a_ld] = 2.f * w * d * inv_batch;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(256) sum_partials_kernel(const float* __restrict__ partial,
+                                                           int32_t n, float* __restrict__ out) {
+  __shared__ float red[256];
+  float acc = 0.f;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += partial[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) red[threadIdx.x] += red[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+}  // namespace
+
+extern "C" {
+
+int geo_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs,
+                  int64_t a_cs, const uint16_t* B, int64_t b_rs, int64_t b_cs, const float* bias,
+                  float* c_f32, uint16_t* c_bf16, int64_t ldc, int32_t accumulate, void* stream) {
+  if (M < 1 || N < 1 || K < 1 || !A || !B || (!c_f32 && !c_bf16) || ldc < N)
+    return geo_internal_fail(GEO_ERR_INVALID, "gemm: bad sizes or null operands");
+  if (accumulate && !c_f32)
+    return geo_internal_fail(GEO_ERR_INVALID, "gemm: accumulation needs the FP32 output");
+  const int64_t gy = (M + kTM - 1) / kTM, gx = (N + kTN - 1) / kTN;
+  if (gy > 65535 || gx > 2147483647)
+    return geo_internal_fail(GEO_ERR_INVALID, "gemm: more than 65535 row tiles");
+  hipLaunchKernelGGL(gemm_bf16_kernel, dim3((uint32_t)gx, (uint32_t)gy), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), M, N, K, A, a_rs, a_cs, B, b_rs, b_cs,
+                     bias, c_f32, c_bf16, ldc, accumulate);
+  SG_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+int geo_f32_to_bf16(int64_t n, const float* x, uint16_t* out, void* stream) {
+  if (n < 0 || (n && (!x || !out))) return geo_internal_fail(GEO_ERR_INVALID, "null argument");
+  if (n == 0) return GEO_OK;
+  hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(grid_for(n)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), n, x, out);
+  SG_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+int geo_silu_forward(int64_t n, const float* pre, uint16_t* out_bf16, void* stream) {
+  if (n < 0 || (n && (!pre || !out_bf16))) return geo_internal_fail(GEO_ERR_INVALID, "null argument");
+  if (n == 0) return GEO_OK;
+  hipLaunchKernelGGL(silu_forward_kernel, dim3(grid_for(n)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), n, pre, out_bf16);
+  SG_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+int geo_silu_backward(int64_t n, const float* pre, const float* dout, float* dpre,
+                      uint16_t* dpre_bf16, void* stream) {
+  if (n < 0 || (n && (!pre || !dout || !dpre)))
+    return geo_internal_fail(GEO_ERR_INVALID, "null argument");
+  if (n == 0) return GEO_OK;
+  hipLaunchKernelGGL(silu_backward_kernel, dim3(grid_for(n)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), n, pre, dout, dpre, dpre_bf16);
+  SG_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+int geo_column_sums(int64_t M, int64_t N, const float* x, int64_t ld, float* out, void* stream) {
+  if (M < 1 || N < 1 || ld < N || !x || !out)
+    return geo_internal_fail(GEO_ERR_INVALID, "column sums: bad sizes");
+  hipLaunchKernelGGL(column_sums_kernel, dim3((uint32_t)((N + 63) / 64)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), M, N, x, ld, out);
+  SG_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+int geo_im2col_up(int64_t batch, int64_t length, int64_t channels, int32_t up, int32_t ksize,
+                  const uint16_t* x, uint16_t* cols, void* stream) {
+  if (batch < 1 || length < 1 || channels < 1 || up < 1 || ksize < 1 || !(ksize & 1) || !x ||
+      !cols)
+    return geo_internal_fail(GEO_ERR_INVALID, "im2col: bad sizes (odd ksize required)");
+  const int64_t total = batch * length * up * ksize * channels;
+  hipLaunchKernelGGL(im2col_up_kernel, dim3(grid_for(total)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), batch, length, channels, up, ksize, x,
+                     cols);
+  SG_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+int geo_col2im_up(int64_t batch, int64_t length, int64_t channels, int32_t up, int32_t ksize,
+                  const float* dcols, float* dx, void* stream) {
+  if (batch < 1 || length < 1 || channels < 1 || up < 1 || ksize < 1 || !(ksize & 1) ||
+      !dcols || !dx)
+    return geo_internal_fail(GEO_ERR_INVALID, "col2im: bad sizes (odd ksize required)");
+  const int64_t total = batch * length * channels;
+  hipLaunchKernelGGL(col2im_up_kernel, dim3(grid_for(total)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), batch, length, channels, up, ksize, dcols,
+                     dx);
+  SG_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+int geo_adam_step(int64_t n, float* param, const float* grad, float* m, float* v,
+                  uint16_t* param_bf16, float lr, float beta1, float beta2, float eps,
+                  int64_t t, void* stream) {
+  if (n < 1 || !param || !grad || !m || !v || t < 1)
+    return geo_internal_fail(GEO_ERR_INVALID, "adam: bad arguments");
+  const float c1 = 1.f - powf(beta1, (float)t), c2 = 1.f - powf(beta2, (float)t);
+  hipLaunchKernelGGL(adam_kernel, dim3(grid_for(n)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), n, param, grad, m, v, param_bf16, lr,
+                     beta1, beta2, eps, c1, c2);
+  SG_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+int geo_trace_loss(int64_t batch, int64_t length, const float* shape, const float* shape_t,
+                   const float* amp, int64_t amp_ld, const float* amp_t, float amp_weight,
+                   float* d_shape, float* d_amp, float* partial, float* loss_out,
+                   void* stream) {
+  if (batch < 1 || length < 1 || amp_ld < 1 || !shape || !shape_t || !amp || !amp_t ||
+      !d_shape || !d_amp || !partial || !loss_out)
+    return geo_internal_fail(GEO_ERR_INVALID, "trace loss: bad arguments");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(trace_loss_kernel, dim3(kLossBlocks), dim3(256), 0, s, batch, length, shape,
+                     shape_t, amp, amp_ld, amp_t, amp_weight, d_shape, d_amp, partial);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, s, (const float*)partial,
+                     kLossBlocks, loss_out);
+  SG_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+}  // extern "C"

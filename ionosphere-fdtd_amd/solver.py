@@ -294,16 +294,20 @@ class GeodesicFDTD:
     def _prepare_material_coefficients(self) -> None:
         """FP64 exponential (or trapezoidal) loss coefficients (solver.py:706-761)."""
         dt = self.time_step_s
-        eps_er = EPSILON_0 * self.epsilon_r_er
-        eps_et = EPSILON_0 * self.epsilon_r_et
+        # Horizontally uniform materials (every benchmark profile) are computed
+        # on one radial row: the same per-element arithmetic on fewer elements.
+        sig_er, rel_er = _shared_rows(self.sigma_er, self.epsilon_r_er)
+        sig_et, rel_et = _shared_rows(self.sigma_et, self.epsilon_r_et)
+        eps_er = EPSILON_0 * rel_er
+        eps_et = EPSILON_0 * rel_et
         if self.config.loss_integration == "trapezoidal":
-            l_er = self.sigma_er * dt / (2.0 * eps_er)
-            l_et = self.sigma_et * dt / (2.0 * eps_et)
+            l_er = sig_er * dt / (2.0 * eps_er)
+            l_et = sig_et * dt / (2.0 * eps_et)
             ca_er, cb_er = (1.0 - l_er) / (1.0 + l_er), dt / (eps_er * (1.0 + l_er))
             ca_et, cb_et = (1.0 - l_et) / (1.0 + l_et), dt / (eps_et * (1.0 + l_et))
         else:
-            ca_er, cb_er = exponential_loss_coefficients(self.sigma_er, eps_er, dt)
-            ca_et, cb_et = exponential_loss_coefficients(self.sigma_et, eps_et, dt)
+            ca_er, cb_er = exponential_loss_coefficients(sig_er, eps_er, dt)
+            ca_et, cb_et = exponential_loss_coefficients(sig_et, eps_et, dt)
         if self.config.compress_uniform_material_coefficients:
             ca_er = self._uniform_radial_profile(ca_er, "radial electric")
             cb_er = self._uniform_radial_profile(cb_er, "radial electric")
@@ -315,6 +319,8 @@ class GeodesicFDTD:
 
     @staticmethod
     def _uniform_radial_profile(values, label):
+        if values.shape[0] == 1:
+            return values
         profile = values[:1]
         if not np.array_equal(values, np.broadcast_to(profile, values.shape)):
             raise ValueError(f"cannot compress horizontally varying {label} coefficients")
@@ -949,6 +955,13 @@ def stable_time_step(config: SimulationConfig, mesh, material=None, plasma=None)
                                        config.earth_radius_m)
     limit = cfl_time_step_limit(mesh, radii, np.diff(radii), e_er, e_et, plasma)
     return config.time_step_s if config.time_step_s is not None else config.courant_factor * limit
+
+
+def _shared_rows(*arrays):
+    """The arrays' first rows if every array repeats its first row, else the arrays."""
+    if all(a.shape[0] > 0 and np.array_equal(a, np.broadcast_to(a[:1], a.shape)) for a in arrays):
+        return tuple(a[:1] for a in arrays)
+    return arrays
 
 
 def torch_any_nonzero(t) -> bool:

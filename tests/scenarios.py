@@ -264,7 +264,9 @@ def build_sim(sc: dict, *, compile_step=False, compile_chunk_size=8, dtype=None,
                   storage_scaling=storage_scaling)
     from ionosphere_fdtd_amd.solver import stable_time_step
 
-    dt = stable_time_step(cfg, mesh, material, plasma)
+    timed = [v for part in (sc["source"] or {}, sc["tangential"] or {}) for v in part.values()
+             if isinstance(v, str) and v.endswith("dt")]
+    dt = stable_time_step(cfg, mesh, material, plasma) if timed else None
     source = None
     if sc["source"]:
         s = {k: _resolve_time(v, dt) for k, v in sc["source"].items()}

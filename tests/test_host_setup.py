@@ -193,8 +193,8 @@ def test_config_validation_mirrors_reference():
 
 
 def test_library_exports_every_declared_symbol():
-    header = (REPO / "include" / "geofdtd.h").read_text()
-    declared = set(re.findall(r"\b(geo_[a-z_]+)\s*\(", header))
+    header = "".join(p.read_text() for p in sorted((REPO / "include").glob("*.h")))
+    declared = set(re.findall(r"\b(geo_[a-z_0-9]+)\s*\(", header))
     assert declared == set(_native.EXPORTED_SYMBOLS)
     lib = _native.load_library()
     for name in declared:

@@ -332,3 +332,74 @@ def test_planted_oracle_mistakes_are_caught(name, plant):
     errs = _compare_to_fixture(sim, fx, 1e-12)
     errs["traces"] = S.rel_l2(traces, fx["traces"])
     assert max(errs.values()) > 1e-6, errs
+
+
+# ---------------------------------------------------------------------------
+# Trace surrogate oracle (no reference counterpart): pinned by finite
+# differences and by the adjoint / convolution identities of its layout ops.
+# ---------------------------------------------------------------------------
+def _surrogate_oracle():
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location(
+        "surrogate_oracle", S.ORACLE_DIR / "surrogate_oracle.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def test_surrogate_im2col_is_upsampled_convolution_and_col2im_its_adjoint():
+    so = _surrogate_oracle()
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, 6, 3))
+    cols = so.im2col_up(x, 2, 5)
+    assert cols.shape == (2 * 12, 15)
+    # one output channel with weights w equals np.convolve on the upsampled signal
+    w = rng.standard_normal((5, 3))
+    out = (cols @ w.reshape(15, 1)).reshape(2, 12)
+    up = np.repeat(x, 2, axis=1)
+    for b in range(2):
+        want = sum(np.convolve(up[b, :, c], w[::-1, c], mode="same") for c in range(3))
+        np.testing.assert_allclose(out[b], want, rtol=1e-12, atol=1e-12)
+    y = rng.standard_normal(cols.shape)
+    lhs = float(np.sum(cols * y))
+    rhs = float(np.sum(x * so.col2im_up(y, 2, 6, 3, 2, 5)))
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_surrogate_oracle_gradients_match_finite_differences():
+    so = _surrogate_oracle()
+    from ionosphere_fdtd_amd.surrogate import SurrogateConfig, initial_parameters
+
+    cfg = SurrogateConfig(in_features=5, width=7, depth=2, seq0=4, channels=(3, 2, 2), ksize=3,
+                          amp_weight=0.7)
+    rng = np.random.default_rng(11)
+    params = {k: v.astype(np.float64) for k, v in initial_parameters(cfg, seed=3).items()}
+    for k in params:
+        if not k.startswith("W"):
+            params[k] = 0.1 * rng.standard_normal(params[k].shape)
+    x = rng.standard_normal((3, 5))
+    st = rng.standard_normal((3, cfg.length))
+    at = rng.standard_normal(3)
+
+    def loss_of(p):
+        shape, amp, _ = so.forward(p, x, cfg, False)
+        return so.trace_loss(shape, amp, st, at, cfg.amp_weight)[0]
+
+    shape, amp, cache = so.forward(params, x, cfg, False)
+    _, ds, da = so.trace_loss(shape, amp, st, at, cfg.amp_weight)
+    grads = so.backward(cache, ds, da, cfg, False)
+    assert set(grads) == set(params)
+    eps = 1e-6
+    for name, value in params.items():
+        flat = value.reshape(-1)
+        for idx in rng.choice(flat.size, size=min(6, flat.size), replace=False):
+            keep = flat[idx]
+            flat[idx] = keep + eps
+            up = loss_of(params)
+            flat[idx] = keep - eps
+            down = loss_of(params)
+            flat[idx] = keep
+            fd = (up - down) / (2 * eps)
+            g = grads[name].reshape(-1)[idx]
+            assert abs(fd - g) <= 1e-6 * max(1.0, abs(fd)), (name, idx, fd, g)
