@@ -240,6 +240,67 @@ def ensemble_bench(args) -> dict:
     }
 
 
+def surrogate_bench(args) -> dict:
+    """Learned surrogate with FDTD fallback on held-out trace queries.
+
+    2000 held-out queries (250 seeded C2-style configurations x 8 stations)
+    are answered (a) by batched plain FDTD -- the ground truth and the time
+    to beat -- and (b) by the query service: the trained network for
+    in-range queries, one spot-checked configuration simulated per batch
+    (any error > 2 % sends the batch to FDTD), FDTD for out-of-range queries.
+    A further 16 out-of-range queries exercise the fallback."""
+    import numpy as np
+    import torch
+
+    import ionosphere_fdtd_amd as g
+    from ionosphere_fdtd_amd import trace_queries as tq
+
+    model_path = REPO / "ionosphere-fdtd_amd" / "data" / "trace_surrogate.npz"
+    model = tq.SurrogateModel.load(model_path)
+    mesh = g.build_geodesic_mesh(tq.SUBDIVISION)
+    runs = tq.sample_runs(args.queries // len(tq.STATION_OFFSETS), seed=20261204)
+    queries = tq.station_queries(runs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    truth = tq.fdtd_traces(queries, mesh)
+    torch.cuda.synchronize()
+    t_fdtd = time.perf_counter() - t0
+    print(f"batched FDTD: {len(queries)} queries in {t_fdtd:.1f} s", file=sys.stderr, flush=True)
+    service = tq.TraceQueryService(model, mesh, tolerance=0.02, spot_checks=1, seed=7)
+    service.answer(queries[:8])  # warm-up (graphs, kernels)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    answers, served, report = service.answer(queries)
+    torch.cuda.synchronize()
+    t_service = time.perf_counter() - t1
+    err = tq.trace_rel_l2(answers, truth)
+    # fallback: out-of-range physics and a receiver off the station network
+    odd = queries[:16].copy()
+    odd[:8, 1] = 0.7
+    odd[8:, 5] += 0.5
+    o_ans, o_served, o_report = service.answer(odd)
+    o_truth = tq.fdtd_traces(odd, mesh)
+    return {
+        "metric": "ensemble trace queries/s at <=2% err (surrogate + FDTD fallback)",
+        "value": round(len(queries) / t_service, 1), "unit": "queries/s",
+        "n_gpus": 1, "higher_is_better": True, "dtype": "bf16 network / f32 FDTD",
+        "data": "synthetic", "vs_baseline": None,
+        "config": {"workload": "2000 held-out C2-style station queries (s5, R=32, 4096 steps, "
+                               "512-sample Er traces)", "model": str(model_path.name),
+                   "train": "4096 runs x 8 stations, 6000 Adam steps"},
+        "batched_fdtd_queries_per_s": round(len(queries) / t_fdtd, 2),
+        "speedup_vs_batched_fdtd": round(t_fdtd / t_service, 1),
+        "served_by_surrogate": int(np.sum(served == "surrogate")),
+        "served_by_fdtd": int(np.sum(served == "fdtd")),
+        "spot_check_errors": report.get("spot_check_errors"),
+        "error_vs_fdtd": {"median": float(np.median(err)), "p90": float(np.quantile(err, 0.9)),
+                          "p99": float(np.quantile(err, 0.99)), "max": float(err.max()),
+                          "within_2pct": float(np.mean(err <= 0.02))},
+        "fallback_check": {"queries": len(odd), "served_by_fdtd": int(np.sum(o_served == "fdtd")),
+                           "max_error_vs_fdtd": float(tq.trace_rel_l2(o_ans, o_truth).max())},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -250,7 +311,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="c1",
                     help="c1 (default), c1_noplasma, c1_pec, c2 (64-run ensemble), c3, "
-                         "vacuum_s7_r64, vacuum_s7_r80")
+                         "c4 (surrogate + FDTD fallback), vacuum_s7_r64, vacuum_s7_r80")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="also time a 1 GiB device copy (read + write GB/s)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -260,6 +321,7 @@ def main() -> int:
                          "refined 60-90 km band) for --steps steps instead of the throughput "
                          "bench")
     ap.add_argument("--runs", type=int, default=64, help="ensemble runs (--workload c2)")
+    ap.add_argument("--queries", type=int, default=2000, help="held-out queries (--workload c4)")
     ap.add_argument("--sweep-dtypes", default="float32,float16,bfloat16",
                     help="storage types compared with the FP64 run in --precision-sweep")
     args = ap.parse_args()
@@ -275,6 +337,9 @@ def main() -> int:
     torch.cuda.set_device(0)
     if args.workload == "c2":
         print(json.dumps(ensemble_bench(args)))
+        return 0
+    if args.workload == "c4":
+        print(json.dumps(surrogate_bench(args)))
         return 0
     if args.precision_sweep:
         sweep = precision_sweep(args)

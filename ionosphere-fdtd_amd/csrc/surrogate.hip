@@ -48,10 +48,14 @@ __device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + __expf
 // ---------------------------------------------------------------------------
 constexpr int kTM = 64, kTN = 64, kTK = 32, kLdsRow = kTK + 8;
 
+// Split K: block z covers k in [z * k_chunk, (z + 1) * k_chunk) and, with
+// more than one split, writes its raw partial product to partial[z][m][n];
+// gemm_reduce_kernel then adds the partials in split order (deterministic).
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(
     int64_t M, int64_t N, int64_t K, const uint16_t* __restrict__ A, int64_t a_rs, int64_t a_cs,
     const uint16_t* __restrict__ B, int64_t b_rs, int64_t b_cs, const float* __restrict__ bias,
-    float* __restrict__ c32, uint16_t* __restrict__ c16, int64_t ldc, int32_t accumulate) {
+    float* __restrict__ c32, uint16_t* __restrict__ c16, int64_t ldc, int32_t accumulate,
+    int64_t k_chunk, float* __restrict__ partial) {
   __shared__ __attribute__((aligned(16))) uint16_t As[kTM * kLdsRow];
   __shared__ __attribute__((aligned(16))) uint16_t Bt[kTN * kLdsRow];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -65,14 +69,16 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(
   // loader coordinates: tile row (0..63) and 8 consecutive k
   const int lr = t >> 2, lk = (t & 3) * 8;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
-  for (int64_t k0 = 0; k0 < K; k0 += kTK) {
+  const int64_t k_begin = (int64_t)blockIdx.z * k_chunk;
+  const int64_t k_end = k_begin + k_chunk < K ? k_begin + k_chunk : K;
+  for (int64_t k0 = k_begin; k0 < k_end; k0 += kTK) {
     {
       const int64_t m = m0 + lr;
       uint16_t v[8];
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int64_t k = k0 + lk + e;
-        v[e] = (m < M && k < K) ? A[m * a_rs + k * a_cs] : (uint16_t)0;
+        v[e] = (m < M && k < k_end) ? A[m * a_rs + k * a_cs] : (uint16_t)0;
       }
       uint4 packed;
       packed.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
@@ -87,7 +93,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(
 #pragma unroll
       for (int e = 0; e < 8; ++e) {
         const int64_t k = k0 + lk + e;
-        v[e] = (n < N && k < K) ? B[k * b_rs + n * b_cs] : (uint16_t)0;
+        v[e] = (n < N && k < k_end) ? B[k * b_rs + n * b_cs] : (uint16_t)0;
       }
       uint4 packed;
       packed.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
@@ -125,6 +131,10 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(
         const int64_t n = n0 + wn + j * 16 + r16;
         if (m >= M || n >= N) continue;
         float v = acc[i][j][r];
+        if (partial != nullptr) {
+          partial[((int64_t)blockIdx.z * M + m) * N + n] = v;
+          continue;
+        }
         if (bias != nullptr) v += bias[n];
         if (c32 != nullptr) {
           float* p = c32 + m * ldc + n;
@@ -132,6 +142,25 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(
         }
         if (c16 != nullptr) c16[m * ldc + n] = f2bf(v);
       }
+}
+
+__global__ void gemm_reduce_kernel(int64_t M, int64_t N, int32_t splits,
+                                   const float* __restrict__ partial, const float* __restrict__ bias,
+                                   float* __restrict__ c32, uint16_t* __restrict__ c16,
+                                   int64_t ldc, int32_t accumulate) {
+  const int64_t total = M * N;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = i / N, n = i - m * N;
+    float v = 0.f;
+    for (int32_t z = 0; z < splits; ++z) v += partial[(int64_t)z * total + i];
+    if (bias != nullptr) v += bias[n];
+    if (c32 != nullptr) {
+      float* p = c32 + m * ldc + n;
+      *p = accumulate ? *p + v : v;
+    }
+    if (c16 != nullptr) c16[m * ldc + n] = f2bf(v);
+  }
 }
 
 int64_t grid_for(int64_t n, int threads = 256, int64_t cap = 65536) {
@@ -297,10 +326,34 @@ int geo_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_
   const int64_t gy = (M + kTM - 1) / kTM, gx = (N + kTN - 1) / kTN;
   if (gy > 65535 || gx > 2147483647)
     return geo_internal_fail(GEO_ERR_INVALID, "gemm: more than 65535 row tiles");
-  hipLaunchKernelGGL(gemm_bf16_kernel, dim3((uint32_t)gx, (uint32_t)gy), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), M, N, K, A, a_rs, a_cs, B, b_rs, b_cs,
-                     bias, c_f32, c_bf16, ldc, accumulate);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // Few output tiles over a long K (weight gradients: K = batch x length):
+  // split K so that the launch fills the chip, at most 256 splits of >= 1024.
+  int64_t splits = 1;
+  if (gx * gy < 512 && K >= 2048) {
+    splits = 512 / (gx * gy);
+    if (splits > K / 1024) splits = K / 1024;
+    if (splits > 256) splits = 256;
+    if (splits < 1) splits = 1;
+  }
+  const int64_t k_chunk = ((K + splits - 1) / splits + kTK - 1) / kTK * kTK;
+  splits = (K + k_chunk - 1) / k_chunk;
+  if (splits == 1) {
+    hipLaunchKernelGGL(gemm_bf16_kernel, dim3((uint32_t)gx, (uint32_t)gy, 1), dim3(256), 0, s, M,
+                       N, K, A, a_rs, a_cs, B, b_rs, b_cs, bias, c_f32, c_bf16, ldc, accumulate,
+                       K, (float*)nullptr);
+    SG_HIP(hipGetLastError());
+    return GEO_OK;
+  }
+  float* partial = nullptr;
+  SG_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(float) * splits * M * N, s));
+  hipLaunchKernelGGL(gemm_bf16_kernel, dim3((uint32_t)gx, (uint32_t)gy, (uint32_t)splits),
+                     dim3(256), 0, s, M, N, K, A, a_rs, a_cs, B, b_rs, b_cs, (const float*)nullptr,
+                     (float*)nullptr, (uint16_t*)nullptr, ldc, 0, k_chunk, partial);
+  hipLaunchKernelGGL(gemm_reduce_kernel, dim3(grid_for(M * N)), dim3(256), 0, s, M, N,
+                     (int32_t)splits, (const float*)partial, bias, c_f32, c_bf16, ldc, accumulate);
   SG_HIP(hipGetLastError());
+  SG_HIP(hipFreeAsync(partial, s));
   return GEO_OK;
 }
 

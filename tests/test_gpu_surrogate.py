@@ -34,7 +34,8 @@ def _dev16(torch, bits):
 
 @pytest.mark.parametrize("M,N,K,ta,tb", [(70, 45, 37, False, False), (128, 64, 64, True, False),
                                           (1, 1, 1, False, True), (300, 17, 96, True, True),
-                                          (64, 256, 512, False, False)])
+                                          (64, 256, 512, False, False),
+                                          (80, 16, 40000, True, False)])  # split K
 def test_gemm_bf16_matches_fp64_product_of_bf16_operands(M, N, K, ta, tb):
     import torch
 
