@@ -158,7 +158,8 @@ typedef struct GeoState {
   void* ht; int64_t ht_pitch;        /* (E, ht_pitch) */
   void* et; int64_t et_pitch;        /* (E, et_pitch) */
   void* hr; int64_t hr_pitch;        /* (F, hr_pitch) */
-  void* si_memory;                   /* (si_terms, E) term-major, compute type */
+  void* si_memory;                   /* (E, si_pitch) edge-major, compute type; columns past
+                                        si_terms are zero */
   void* plasma_current;              /* (n_species, components, F, plasma_pitch), compute */
   int64_t plasma_pitch;
 } GeoState;
@@ -174,6 +175,7 @@ typedef struct GeoLayout {
                                 3 = Cartesian J (magnetised) */
   int64_t vector_elems;    /* elements per 16-byte strip */
   int64_t compute_bytes;   /* 4 (FP32 arithmetic) or 8 (FP64) */
+  int64_t si_pitch;        /* row pitch of the surface-impedance memory (16-byte rows) */
 } GeoLayout;
 
 typedef struct GeoDiagOut {

@@ -81,7 +81,8 @@ class GeoLayout(C.Structure):
     _fields_ = [("node_pitch", C.c_int64), ("mid_pitch", C.c_int64), ("band_lo", C.c_int64),
                 ("band_hi", C.c_int64), ("plasma_pitch", C.c_int64),
                 ("plasma_components", C.c_int64),
-                ("vector_elems", C.c_int64), ("compute_bytes", C.c_int64)]
+                ("vector_elems", C.c_int64), ("compute_bytes", C.c_int64),
+                ("si_pitch", C.c_int64)]
 
 
 class GeoDiagOut(C.Structure):

@@ -794,6 +794,7 @@ struct Plan final : PlanBase {
     hp.si_scale = d_sis;
     hp.si_scale_stride = (si_on && ph->si_scale_rows == E) ? 1 : 0;
     hp.si_terms = si_terms;
+    hp.si_pitch = (int32_t)round_up(si_terms, 16 / (int64_t)sizeof(C));
     if (si_on) {
       const double cs = 2.0 * dt / (kMu0 * dr[0]);
       const double metric = full ? mid[0] / rad[0] : 1.0;
@@ -816,7 +817,6 @@ struct Plan final : PlanBase {
     {
       std::vector<SegSpec> segs = {{geo::kSegHEdge, geom_blocks(hp.g_edge)},
                                    {geo::kSegHFace, geom_blocks(hp.g_face)}};
-      if (si_on) segs.push_back({geo::kSegSi, (int32_t)((E + geo::kBlock - 1) / geo::kBlock)});
       segs.push_back({geo::kSegSample, 1});
       std::vector<uint32_t> map = build_block_map(segs);
       uint32_t* d_map;
@@ -939,6 +939,7 @@ struct Plan final : PlanBase {
     layout.plasma_components = ncomp;
     layout.vector_elems = N;
     layout.compute_bytes = sizeof(C);
+    layout.si_pitch = si_on ? hp.si_pitch : 0;
     GEO_HIP(hipStreamCreateWithFlags(&capture_stream, hipStreamNonBlocking));
     return GEO_OK;
   }
@@ -1402,7 +1403,6 @@ struct Ensemble final : EnsembleBase {
       const Plan<T, C>& p = *plans[r];
       std::vector<SegSpec> hs = {{geo::kSegHEdge, geom_blocks(host_hp[r].g_edge)},
                                  {geo::kSegHFace, geom_blocks(host_hp[r].g_face)}};
-      if (p.si_on) hs.push_back({geo::kSegSi, (int32_t)((p.E + geo::kBlock - 1) / geo::kBlock)});
       hs.push_back({geo::kSegSample, 1});
       hm[r] = build_block_map(hs);
       em[r] = build_block_map({{geo::kSegEVertex, geom_blocks(host_ep[r].g_vertex)},
@@ -1516,7 +1516,7 @@ struct Ensemble final : EnsembleBase {
       v.push_back({(void*)p->hp.ht, sizeof(T) * p->E * p->pn});
       v.push_back({(void*)p->hp.et, sizeof(T) * p->E * p->pm});
       v.push_back({(void*)p->hp.hr, sizeof(T) * p->F * p->pm});
-      if (p->si_on) v.push_back({(void*)p->hp.si_mem, sizeof(C) * p->si_terms * p->E});
+      if (p->si_on) v.push_back({(void*)p->hp.si_mem, sizeof(C) * p->hp.si_pitch * p->E});
       if (p->plasma)
         v.push_back({(void*)p->hp.pl.current, sizeof(C) * p->hp.pl.n_species *
                                                   p->layout.plasma_components * p->F *

@@ -35,7 +35,7 @@ constexpr int kMaxSpecies = 4;
 constexpr int kNumXcd = 8;
 
 enum Segment : uint32_t {
-  kSegHEdge = 0, kSegHFace = 1, kSegSi = 2, kSegSample = 3,
+  kSegHEdge = 0, kSegHFace = 1, kSegSample = 3,
   kSegEVertex = 4, kSegEEdge = 5,
 };
 
@@ -309,9 +309,11 @@ struct HParams {
   const C* __restrict__ rd_b;             // (pn,) dt/mu0 alpha_k
   const C* __restrict__ rd_c;             // (pn,) dt/mu0 gamma_k
   const C* __restrict__ inv_rbar;         // (pm,)
-  // surface impedance
+  // surface impedance: memory edge-major, (E, si_pitch), each edge's terms one
+  // contiguous 16-byte-aligned row read and written by the thread that owns
+  // the edge's first radial strip
   C* __restrict__ si_mem;
-  // memory stored term-major, (terms, E), so every term access is coalesced
+  int32_t si_pitch;
   const C* __restrict__ si_scale;         // (rows,) sqrt(mu/sigma)
   int32_t si_scale_stride;                // 0 or 1
   int32_t si_terms;
@@ -398,6 +400,7 @@ __device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, const Scale
   load_vec<N>(p.rd_b + k0, rb);
   load_vec<N>(p.rd_c + k0, rc);
   load_vec<N>(p.inv_r + k0, ir);
+  const C h_old0 = ht[0];
   // Rows are padded with zeros and the radial tables are zero beyond R, so
   // the strip is updated unconditionally: Et(R) reads the zero padding (odd
   // ghost, PEC cap, solver.py:448-449) and padding lanes stay unchanged.
@@ -408,13 +411,9 @@ __device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, const Scale
     ht[i] = ht[i] + a * ir[i] * (erh[i] - ert[i]) - rd;
     prev = cur;
   }
-  if (p.si_on && k0 == 0) {
-    // element 0 belongs to the surface-impedance segment
-#pragma unroll
-    for (int i = 1; i < N; ++i) store_one(hrow + i, ht[i], sc.ht);
-  } else {
-    store_strip_stream(hrow, ht, sc.ht);
-  }
+  if (p.si_on && k0 == 0)  // node 0 under a surface-impedance ground replaces the PEC value
+    ht[0] = si_boundary(p, e, a * ir[0] * (erh[0] - ert[0]), h_old0, et[0]);
+  store_strip_stream(hrow, ht, sc.ht);
 }
 
 template <typename C>
@@ -565,54 +564,55 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
   }
 }
 
+// Implicit lower-boundary Ht of edge e with the diffusive surface-impedance
+// memory (surface_impedance.py:225-270), in increment form:
+// ((1 - c/2) h_old + rest) / (1 + c/2) = h_old + (rest - c h_old) / (1 + c/2)
+// and m' = m + (sum drive - (1 - decay) m), so the small coupling c and the
+// slow decay rates of the low-frequency poles keep their relative precision.
+// Returns the new Ht(e, 0); forcing = dt/(mu0 theta r0) (Er_head - Er_tail).
 template <typename T, typename C>
-__device__ __forceinline__ void h_si_edge(const HParams<T, C>& p, const Scales<C>& sc, int32_t e) {
-  // Implicit lower-boundary Ht with 16-pole diffusive memory
-  // (surface_impedance.py:225-270), in increment form:
-  // ((1 - c/2) h_old + rest) / (1 + c/2) = h_old + (rest - c h_old) / (1 + c/2)
-  // and m' = m + (sum drive - (1 - decay) m), so the small coupling c and the
-  // slow decay rates of the low-frequency poles keep their relative precision.
-  const int32_t tail = p.edges[2 * e];
-  const int32_t head = p.edges[2 * e + 1];
-  const C forcing = p.edge_grad[e] * p.inv_r[0] *
-                    (load_one(p.er + (size_t)head * p.pn, sc.er_inv) -
-                     load_one(p.er + (size_t)tail * p.pn, sc.er_inv));
-  T* hp = p.ht + (size_t)e * p.pn;
-  const C h_old = load_one(hp, sc.ht_inv);
-  const C et0 = load_one(p.et + (size_t)e * p.pm, sc.et_inv);
+__device__ __forceinline__ C si_boundary(const HParams<T, C>& p, int32_t e, C forcing, C h_old,
+                                         C et0) {
   const C scale = p.si_scale[e * p.si_scale_stride];
-  C* mem = p.si_mem + e;
-  const size_t ts = (size_t)p.n_edges;
+  C* mem = p.si_mem + (size_t)e * p.si_pitch;
   const C coupling = p.si_cs * (scale * p.si_hw_sum);
-  if (p.si_terms <= kSiFastTerms) {
-    // all terms in registers: the loads are independent and issue back to back
+  constexpr int V = 16 / sizeof(C);
+  if (p.si_pitch <= kSiFastTerms) {
+    // the row in registers: 16-byte loads, terms past si_terms are zero with
+    // zero weights, drives and losses
     C m[kSiFastTerms];
 #pragma unroll
-    for (int i = 0; i < kSiFastTerms; ++i) m[i] = (i < p.si_terms) ? mem[i * ts] : (C)0;
+    for (int i = 0; i < kSiFastTerms; i += V) {
+      if (i < p.si_pitch) load_vec_stream<V>(mem + i, m + i);
+      else
+#pragma unroll
+        for (int q = 0; q < V; ++q) m[i + q] = (C)0;
+    }
     C hist = (C)0;
 #pragma unroll
     for (int i = 0; i < kSiFastTerms; ++i) hist += m[i] * p.si_hw[i];
     hist *= scale;
     const C h = h_old + (forcing - p.si_cs_metric * et0 + p.si_cs * hist - coupling * h_old) /
                             ((C)1 + (C)0.5 * coupling);
-    store_one(hp, h, sc.ht);
     const C sum = h + h_old;
 #pragma unroll
-    for (int i = 0; i < kSiFastTerms; ++i)
-      if (i < p.si_terms) mem[i * ts] = m[i] + (sum * p.si_drive[i] - p.si_loss[i] * m[i]);
-  } else {
-    C hist = (C)0;
-    for (int i = 0; i < p.si_terms; ++i) hist += mem[i * ts] * p.si_hw[i];
-    hist *= scale;
-    const C h = h_old + (forcing - p.si_cs_metric * et0 + p.si_cs * hist - coupling * h_old) /
-                            ((C)1 + (C)0.5 * coupling);
-    store_one(hp, h, sc.ht);
-    const C sum = h + h_old;
-    for (int i = 0; i < p.si_terms; ++i) {
-      const C mi = mem[i * ts];
-      mem[i * ts] = mi + (sum * p.si_drive[i] - p.si_loss[i] * mi);
-    }
+    for (int i = 0; i < kSiFastTerms; ++i) m[i] = m[i] + (sum * p.si_drive[i] - p.si_loss[i] * m[i]);
+#pragma unroll
+    for (int i = 0; i < kSiFastTerms; i += V)
+      if (i < p.si_pitch) store_vec_stream<V>(mem + i, m + i);
+    return h;
   }
+  C hist = (C)0;
+  for (int i = 0; i < p.si_terms; ++i) hist += mem[i] * p.si_hw[i];
+  hist *= scale;
+  const C h = h_old + (forcing - p.si_cs_metric * et0 + p.si_cs * hist - coupling * h_old) /
+                          ((C)1 + (C)0.5 * coupling);
+  const C sum = h + h_old;
+  for (int i = 0; i < p.si_terms; ++i) {
+    const C mi = mem[i];
+    mem[i] = mi + (sum * p.si_drive[i] - p.si_loss[i] * mi);
+  }
+  return h;
 }
 
 template <typename T, typename C>
@@ -666,9 +666,6 @@ __device__ __forceinline__ void h_dispatch(const HParams<T, C>& p, const Scales<
           h_face_strip<T, C, PLASMA, MAGNETIZED>(p, sc, row, strip, lanes_of(p.g_face, col, strip));
       }
     }
-  } else if (seg == kSegSi) {
-    const int32_t e = b * kBlock + (int32_t)threadIdx.x;
-    if (e < p.n_edges) h_si_edge(p, sc, e);
   } else {
     sample_block(p, sc);
   }

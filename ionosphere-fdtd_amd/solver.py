@@ -450,10 +450,10 @@ class GeodesicFDTD:
         self._ht_buf = torch.zeros((mesh.n_edges, pn), dtype=sdt, device=dev)
         self._et_buf = torch.zeros((mesh.n_edges, pm), dtype=sdt, device=dev)
         self._hr_buf = torch.zeros((mesh.n_faces, pm), dtype=sdt, device=dev)
-        # term-major device layout; the reference's (edge, term) array is a view
-        self._si_buf = (torch.zeros((si_terms, mesh.n_edges), dtype=cdt, device=dev)
+        # edge-major (E, si_pitch) rows; the reference's (E, terms) array is a view
+        self._si_buf = (torch.zeros((mesh.n_edges, layout.si_pitch), dtype=cdt, device=dev)
                         if si_terms else None)
-        self._si_memory = self._si_buf.t() if si_terms else None
+        self._si_memory = self._si_buf[:, :si_terms] if si_terms else None
         n_species = ph.n_species
         self._plasma_buf = (torch.zeros((n_species, layout.plasma_components, mesh.n_faces,
                                          layout.plasma_pitch), dtype=cdt, device=dev)
