@@ -293,6 +293,9 @@ def surrogate_bench(args) -> dict:
         "served_by_surrogate": int(np.sum(served == "surrogate")),
         "served_by_fdtd": int(np.sum(served == "fdtd")),
         "spot_check_errors": report.get("spot_check_errors"),
+        "service_seconds": {"total": round(t_service, 4),
+                            "surrogate": round(report.get("surrogate_seconds", 0.0), 4),
+                            "spot_check_fdtd": round(report.get("spot_check_seconds", 0.0), 4)},
         "error_vs_fdtd": {"median": float(np.median(err)), "p90": float(np.quantile(err, 0.9)),
                           "p99": float(np.quantile(err, 0.99)), "max": float(err.max()),
                           "within_2pct": float(np.mean(err <= 0.02))},

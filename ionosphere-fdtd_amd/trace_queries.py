@@ -61,11 +61,16 @@ ELECTRON_MASS_KG = 9.109_383_713_9e-31
 
 def stations() -> np.ndarray:
     """(transmitter, station, lat/lon) receiver positions."""
-    out = np.zeros((len(TRANSMITTERS), len(STATION_OFFSETS), 2))
-    for t, (lat, lon) in enumerate(TRANSMITTERS):
-        for s, (d_km, az) in enumerate(STATION_OFFSETS):
-            out[t, s] = destination_point(lat, lon, az, d_km * 1e3, EARTH_RADIUS_M)
-    return out
+    if _STATIONS is None:
+        out = np.zeros((len(TRANSMITTERS), len(STATION_OFFSETS), 2))
+        for t, (lat, lon) in enumerate(TRANSMITTERS):
+            for s, (d_km, az) in enumerate(STATION_OFFSETS):
+                out[t, s] = destination_point(lat, lon, az, d_km * 1e3, EARTH_RADIUS_M)
+        globals()["_STATIONS"] = out
+    return _STATIONS.copy()
+
+
+_STATIONS = None
 
 
 def sample_runs(n: int, seed: int) -> np.ndarray:
@@ -336,14 +341,18 @@ class TraceQueryService:
         fdtd_rows = np.flatnonzero(~ok)
         if ok.any():
             rows = np.flatnonzero(ok)
+            t1 = time.perf_counter()
             pred = self.model.predict(q[rows], self.mesh)
+            report["surrogate_seconds"] = time.perf_counter() - t1
             keys = [tuple(q[i, [0, 1, 2, 3, 4, 7]]) for i in rows]
             distinct = sorted(set(keys))
             picks = self.rng.choice(len(distinct), size=min(self.spot_checks, len(distinct)),
                                     replace=False)
             chosen = {distinct[p] for p in picks}
             check = np.asarray([k in chosen for k in keys])
+            t2 = time.perf_counter()
             truth = fdtd_traces(q[rows[check]], self.mesh, batch=64)
+            report["spot_check_seconds"] = time.perf_counter() - t2
             errors = trace_rel_l2(pred[check], truth)
             report["spot_check_errors"] = errors.tolist()
             if np.all(errors <= self.tolerance):
