@@ -505,25 +505,21 @@ struct Plan final : PlanBase {
     ctrl[R] = 0.5 * dr[R - 1];
     for (int64_t k = 1; k < R; ++k) ctrl[k] = mid[k] - mid[k - 1];
     const bool full = r->full_spherical != 0;
-    std::vector<C> inv_r(pn, 0), rd_b(pn, 0), rd_c(pn, 0), inv_rbar(pm, 0), rh_a(pm, 0),
-        rh_c(pm, 0);
+    // gamma_k = 1/r_k (interior nodes) and gamma'_j = 1/rbar_j of the
+    // spherical radial derivative are the inverse radii themselves; the
+    // kernels take them from inv_r / inv_rbar.
+    std::vector<C> inv_r(pn, 0), rd_b(pn, 0), inv_rbar(pm, 0), rh_a(pm, 0);
     for (int64_t k = 0; k <= R; ++k) {
       inv_r[k] = (C)(1.0 / rad[k]);
-      double alpha, gamma = 0.0;
+      double alpha;
       if (k == 0) alpha = full ? 2.0 * mid[0] / (dr[0] * rad[0]) : 2.0 / dr[0];
       else if (k == R) alpha = full ? 2.0 * mid[R - 1] / (dr[R - 1] * rad[R]) : 2.0 / dr[R - 1];
-      else {
-        const double d = mid[k] - mid[k - 1];
-        alpha = full ? mid[k - 1] / (d * rad[k]) : 1.0 / d;
-        gamma = full ? 1.0 / rad[k] : 0.0;
-      }
+      else alpha = full ? mid[k - 1] / ((mid[k] - mid[k - 1]) * rad[k]) : 1.0 / (mid[k] - mid[k - 1]);
       rd_b[k] = (C)(dtm * alpha);
-      rd_c[k] = (C)(dtm * gamma);
     }
     for (int64_t j = 0; j < R; ++j) {
       inv_rbar[j] = (C)(1.0 / mid[j]);
       rh_a[j] = (C)(full ? rad[j] / (dr[j] * mid[j]) : 1.0 / dr[j]);
-      rh_c[j] = (C)(full ? 1.0 / mid[j] : 0.0);
     }
     // ---- topology and metric (solver.py:408-442, numpy_backend.py:46-114) ----
     std::vector<int32_t> edges(2 * E), edge_lr(2 * E), face_edges(3 * F), faces(3 * F);
@@ -643,18 +639,36 @@ struct Plan final : PlanBase {
         // Projected state for B = 0: the ADE is linear and isotropic, so the
         // projections the scatter needs (J.n, J.t_c) obey the same update as
         // J with E.n = radial average and E.t_c = sum_c' P[c][c'] Et_c',
-        // P[c][c'] = t_c . R_c' (R = pinv(T).T, plasma.py:548-559).
+        // P[c][c'] = t_c . R_c' (R = pinv(T).T, plasma.py:548-559).  The
+        // three edge tangents of a face are orthogonal to its centre, hence
+        // coplanar: t_2 = w1 t_0 + w2 t_1, so the update advances J.n, J.t_0
+        // and J.t_1 and writes J.t_2 = w1 J.t_0 + w2 J.t_1 without reading it.
         if (!ph->face_tangents) return fail(GEO_ERR_INVALID, "face tangents are required");
-        recon.assign(9 * F, 0);
-        for (int64_t f = 0; f < F; ++f)
-          for (int c = 0; c < 3; ++c)
+        recon.assign(8 * F, 0);
+        for (int64_t f = 0; f < F; ++f) {
+          const double* t = ph->face_tangents + 9 * f;
+          for (int c = 0; c < 2; ++c)
             for (int cp = 0; cp < 3; ++cp) {
               double dot = 0.0;
               for (int d = 0; d < 3; ++d)
-                dot += ph->face_tangents[(f * 3 + c) * 3 + d] *
-                       ph->plasma_reconstruction[(f * 3 + cp) * 3 + d];
-              recon[9 * f + 3 * c + cp] = (C)dot;
+                dot += t[3 * c + d] * ph->plasma_reconstruction[(f * 3 + cp) * 3 + d];
+              recon[8 * f + 3 * c + cp] = (C)dot;
             }
+          // least squares t_2 ~ w1 t_0 + w2 t_1 (exact for coplanar tangents)
+          double g00 = 0, g01 = 0, g11 = 0, b0 = 0, b1 = 0;
+          for (int d = 0; d < 3; ++d) {
+            g00 += t[d] * t[d];
+            g01 += t[d] * t[3 + d];
+            g11 += t[3 + d] * t[3 + d];
+            b0 += t[d] * t[6 + d];
+            b1 += t[3 + d] * t[6 + d];
+          }
+          const double det = g00 * g11 - g01 * g01;
+          if (!(std::fabs(det) > 1e-12 * g00 * g11))
+            return fail(GEO_ERR_INVALID, "face tangents are degenerate");
+          recon[8 * f + 6] = (C)((b0 * g11 - b1 * g01) / det);
+          recon[8 * f + 7] = (C)((b1 * g00 - b0 * g01) / det);
+        }
         // local slot of every edge in its left and right face
         eslots.assign(2 * E, 0);
         for (int64_t e = 0; e < E; ++e)
@@ -715,15 +729,15 @@ struct Plan final : PlanBase {
 #define UP(vec, ptr) \
   if ((rc = arena.upload(vec, &ptr)) != GEO_OK) return rc
     int32_t *d_edges, *d_lr, *d_fe, *d_faces, *d_ve, *d_vo, *d_vf = nullptr;
-    C *d_eg, *d_ed, *d_fc, *d_vw, *d_ir, *d_rdb, *d_rdc, *d_irb, *d_rha, *d_rhc;
+    C *d_eg, *d_ed, *d_fc, *d_vw, *d_ir, *d_rdb, *d_irb, *d_rha;
     C *d_caer, *d_cber, *d_caet, *d_cbet, *d_sis = nullptr;
     C *d_recon = nullptr, *d_center = nullptr, *d_lt = nullptr, *d_rt = nullptr;
     C *d_vfw = nullptr, *d_coef = nullptr, *d_b = nullptr;
     int32_t* d_es = nullptr;
     UP(edges, d_edges); UP(edge_lr, d_lr); UP(face_edges, d_fe); UP(faces, d_faces);
     UP(vedges, d_ve); UP(vorder, d_vo); UP(edge_grad, d_eg); UP(edge_dual, d_ed); UP(face_curl, d_fc);
-    UP(vw, d_vw); UP(inv_r, d_ir); UP(rd_b, d_rdb); UP(rd_c, d_rdc); UP(inv_rbar, d_irb);
-    UP(rh_a, d_rha); UP(rh_c, d_rhc); UP(cl_er, d_caer); UP(cb_er, d_cber);
+    UP(vw, d_vw); UP(inv_r, d_ir); UP(rd_b, d_rdb); UP(inv_rbar, d_irb);
+    UP(rh_a, d_rha); UP(cl_er, d_caer); UP(cb_er, d_cber);
     UP(cl_et, d_caet); UP(cb_et, d_cbet);
     if (si_on) UP(si_scale, d_sis);
     if (plasma) {
@@ -789,7 +803,7 @@ struct Plan final : PlanBase {
     hp.face_curl = d_fc;
     hp.inv_r = d_ir;
     hp.rd_b = d_rdb;
-    hp.rd_c = d_rdc;
+    hp.gamma_scale = full ? (C)dtm : (C)0;
     hp.inv_rbar = d_irb;
     hp.si_scale = d_sis;
     hp.si_scale_stride = (si_on && ph->si_scale_rows == E) ? 1 : 0;
@@ -844,7 +858,7 @@ struct Plan final : PlanBase {
     ep.inv_r = d_ir;
     ep.inv_rbar = d_irb;
     ep.rh_a = d_rha;
-    ep.rh_c = d_rhc;
+    ep.spherical = full ? 1 : 0;
     ep.pl = pl;
     ep.n_src = ph->n_source;
     for (int q = 0; q < ph->n_source; ++q) {

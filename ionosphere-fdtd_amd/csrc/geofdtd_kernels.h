@@ -273,10 +273,13 @@ struct PlasmaTables {
   int32_t pitch;                // J row pitch (compute elements)
   int64_t species_stride;       // elements between species blocks (comps * F * pitch)
   int64_t comp_stride;          // elements between component blocks (F * pitch)
-  // B = 0: (species, 4, F, pitch) holding J.n, J.t0, J.t1, J.t2;
-  // magnetised: (species, 3, F, pitch) Cartesian J.
+  // B = 0: (species, 4, F, pitch) holding J.n, J.t0, J.t1, J.t2 (the face's
+  // edge tangents are coplanar, so J.t2 = w1 J.t0 + w2 J.t1 is written by the
+  // update and never read by it); magnetised: (species, 3, F, pitch) Cartesian J.
   C* current;
-  const C* recon;               // (F, 9): B = 0 P[c][c'] = t_c.R_c'; else R = pinv(T).T
+  // B = 0: (F, 8) = P[0][0..2], P[1][0..2] (P[c][c'] = t_c.R_c'), w1, w2;
+  // magnetised: (F, 9) R = pinv(T).T
+  const C* recon;
   const int32_t* edge_slots;    // (E, 2) slot of the edge in its left / right face
   const C* center;              // (F, 4) unit normal, padded
   const C* left_tan;            // (E, 4) padded
@@ -307,7 +310,7 @@ struct HParams {
   const C* __restrict__ face_curl;        // (F,3) dt/mu0 s theta / Omega
   const C* __restrict__ inv_r;            // (pn,)
   const C* __restrict__ rd_b;             // (pn,) dt/mu0 alpha_k
-  const C* __restrict__ rd_c;             // (pn,) dt/mu0 gamma_k
+  C gamma_scale;                          // dt/mu0 (spherical) or 0: dt/mu0 gamma_k = gamma_scale / r_k inside
   const C* __restrict__ inv_rbar;         // (pm,)
   // surface impedance: memory edge-major, (E, si_pitch), each edge's terms one
   // contiguous 16-byte-aligned row read and written by the thread that owns
@@ -356,7 +359,7 @@ struct EParams {
   const C* __restrict__ inv_r;            // (pn,)
   const C* __restrict__ inv_rbar;         // (pm,)
   const C* __restrict__ rh_a;             // (pm,) alpha'_j
-  const C* __restrict__ rh_c;             // (pm,) gamma'_j
+  int32_t spherical;                      // gamma'_j = 1/rbar_j (spherical) or 0
   PlasmaTables<C> pl;
   // sources: vertical (Er) and tangential (Et)
   int32_t n_src, n_tsrc;
@@ -396,19 +399,21 @@ __device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, const Scale
   }
   const C from_left = __shfl_up(et[N - 1], 1);
   C prev = ln.left ? from_left : ((k0 > 0) ? load_one(erow + k0 - 1, sc.et_inv) : (C)0);
-  C rb[N], rc[N], ir[N];
+  C rb[N], ir[N];
   load_vec<N>(p.rd_b + k0, rb);
-  load_vec<N>(p.rd_c + k0, rc);
   load_vec<N>(p.inv_r + k0, ir);
   const C h_old0 = ht[0];
   // Rows are padded with zeros and the radial tables are zero beyond R, so
   // the strip is updated unconditionally: Et(R) reads the zero padding (odd
   // ghost, PEC cap, solver.py:448-449) and padding lanes stay unchanged.
+  // dt/mu0 gamma_k = gamma_scale / r_k at interior nodes, 0 at k = 0 and R
+  // (the odd ghosts fold into alpha there).
 #pragma unroll
   for (int i = 0; i < N; ++i) {
+    const int32_t k = k0 + i;
+    const C g = (k > 0 && k < p.R) ? p.gamma_scale : (C)0;
     const C cur = et[i];
-    const C rd = rb[i] * (cur - prev) + rc[i] * cur;
-    ht[i] = ht[i] + a * ir[i] * (erh[i] - ert[i]) - rd;
+    ht[i] = ht[i] + (ir[i] * (a * (erh[i] - ert[i]) - g * cur) - rb[i] * (cur - prev));
     prev = cur;
   }
   if (p.si_on && k0 == 0)  // node 0 under a surface-impedance ground replaces the PEC value
@@ -469,9 +474,6 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
       a1[N] = top ? load_one(p.er + (size_t)v1 * p.pn + j0 + N, sc.er_inv) : (C)0;
       a2[N] = top ? load_one(p.er + (size_t)v2 * p.pn + j0 + N, sc.er_inv) : (C)0;
     }
-    C rc[9];
-#pragma unroll
-    for (int q = 0; q < 9; ++q) rc[q] = pl.recon[9 * f + q];
     C rad[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) {
@@ -481,13 +483,17 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
     }
     const int32_t jb0 = j0 - pl.band_lo;
     if constexpr (!MAGNETIZED) {
-      // drive projections: E.n = rad, E.t_c = sum_c' P[c][c'] Et_c'
-      C drive[4][N];
+      // drive projections: E.n = rad, E.t_c = sum_c' P[c][c'] Et_c' (c = 0, 1)
+      C rc[8];
+      load_vec<4>(pl.recon + 8 * f, rc);
+      load_vec<4>(pl.recon + 8 * f + 4, rc + 4);
+      const C w1 = rc[6], w2 = rc[7];
+      C drive[3][N];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
         drive[0][i] = rad[i];
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
+        for (int c = 0; c < 2; ++c)
           drive[1 + c][i] = (rc[3 * c] * et0[i] + rc[3 * c + 1] * et1[i]) + rc[3 * c + 2] * et2[i];
       }
       for (int s = 0; s < pl.n_species; ++s) {
@@ -497,17 +503,29 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
         C loss[N], dre[N];
         load_vec<N>(cf, loss);
         load_vec<N>(cf + 4 * pl.coef_q_stride, dre);
+        C t2[N];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < 3; ++q) {
           C x[N];
           load_vec_stream<N>(jp + q * pl.comp_stride, x);
 #pragma unroll
           for (int i = 0; i < N; ++i)
             x[i] = (j0 + i < pl.band_hi) ? x[i] + (dre[i] * drive[q][i] - loss[i] * x[i]) : x[i];
           store_vec_stream<N>(jp + q * pl.comp_stride, x);
+          if (q == 1) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) t2[i] = w1 * x[i];
+          } else if (q == 2) {
+#pragma unroll
+            for (int i = 0; i < N; ++i) t2[i] += w2 * x[i];
+          }
         }
+        store_vec_stream<N>(jp + 3 * pl.comp_stride, t2);
       }
     } else {
+      C rc[9];
+#pragma unroll
+      for (int q = 0; q < 9; ++q) rc[q] = pl.recon[9 * f + q];
       C nrm[4];
       load_vec<4>(pl.center + 4 * f, nrm);
       C E[3][N];
@@ -854,12 +872,11 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
   ht[N] = ln.right ? from_right : ((j0 + N <= p.R) ? load_one(hrow + j0 + N, sc.ht_inv) : (C)0);
   T* erow = p.et + (size_t)e * p.pm + j0;
   load_strip(erow, et, sc.et_inv);
-  C cl[N], cb[N], irb[N], ra[N], rcc[N];
+  C cl[N], cb[N], irb[N], ra[N];
   load_vec<N>(p.cl_et + (size_t)e * p.ca_et_stride + j0, cl);
   load_vec<N>(p.cb_et + (size_t)e * p.ca_et_stride + j0, cb);
   load_vec<N>(p.inv_rbar + j0, irb);
   load_vec<N>(p.rh_a + j0, ra);
-  load_vec<N>(p.rh_c + j0, rcc);
   C jt[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) jt[i] = (C)0;
@@ -898,7 +915,7 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
 #pragma unroll
   for (int i = 0; i < N; ++i) {
     const C grad = irb[i] * inv_phi * (hl[i] - hrr[i]);
-    const C rd = ra[i] * (ht[i + 1] - ht[i]) + rcc[i] * ht[i + 1];
+    const C rd = ra[i] * (ht[i + 1] - ht[i]) + (p.spherical ? irb[i] : (C)0) * ht[i + 1];
     et[i] = et[i] + (cb[i] * (grad - rd - jt[i] - src[i]) - cl[i] * et[i]);
   }
   store_strip(erow, et, sc.et);
