@@ -801,7 +801,33 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Sca
       load_vec<4>(pl.vertex_faces + 8 * v, vf);
       load_vec<4>(pl.vertex_faces + 8 * v + 4, vf + 4);
       load_vec<8>(pl.vertex_face_w + 8 * v, vfw);
-      for (int s = 0; s < 6; ++s) {
+      if (!MAGNETIZED && pl.n_species == 1) {
+        // One species: the six face strips are loaded together (padding
+        // slots name face 0 with weight 0), and the weighted sum at midpoint
+        // k0 - 1 is the left lane's last sum (same vertex, same weights, same
+        // order of additions).
+        const int32_t jb0 = k0 - pl.band_lo;
+        if (k0 >= pl.band_lo && k0 < pl.band_hi) {
+          C x[6][N];
+#pragma unroll
+          for (int s = 0; s < 6; ++s) load_vec<N>(pl.current + (size_t)vf[s] * pl.pitch + jb0, x[s]);
+#pragma unroll
+          for (int s = 0; s < 6; ++s)
+#pragma unroll
+            for (int i = 0; i < N; ++i) rv[i + 1] += x[s][i] * vfw[s];
+        }
+        const C from_left = __shfl_up(rv[N], 1);
+        const bool has_prev = k0 - 1 >= pl.band_lo && k0 - 1 < pl.band_hi;
+        if (ln.left && has_prev) {
+          rv[0] = from_left;
+        } else if (has_prev) {
+          C y[6];
+#pragma unroll
+          for (int s = 0; s < 6; ++s) y[s] = pl.current[(size_t)vf[s] * pl.pitch + jb0 - 1];
+#pragma unroll
+          for (int s = 0; s < 6; ++s) rv[0] += y[s] * vfw[s];
+        }
+      } else for (int s = 0; s < 6; ++s) {
         if (vfw[s] == (C)0) continue;
         C strip_v[N], prev;
         if constexpr (MAGNETIZED) {
@@ -892,8 +918,14 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
         plasma_project<C, N, false>(pl, fr, j0, tr, sr, &unused);
       } else {
         const int32_t slot_l = pl.edge_slots[2 * e], slot_r = pl.edge_slots[2 * e + 1];
-        plasma_component<C, N, false>(pl, fl, 1 + slot_l, j0, sl, &unused);
-        plasma_component<C, N, false>(pl, fr, 1 + slot_r, j0, sr, &unused);
+        if (pl.n_species == 1) {  // both sides loaded together
+          const int32_t jb0 = j0 - pl.band_lo;
+          load_vec<N>(pl.current + (1 + slot_l) * pl.comp_stride + (size_t)fl * pl.pitch + jb0, sl);
+          load_vec<N>(pl.current + (1 + slot_r) * pl.comp_stride + (size_t)fr * pl.pitch + jb0, sr);
+        } else {
+          plasma_component<C, N, false>(pl, fl, 1 + slot_l, j0, sl, &unused);
+          plasma_component<C, N, false>(pl, fr, 1 + slot_r, j0, sr, &unused);
+        }
       }
 #pragma unroll
       for (int i = 0; i < N; ++i) jt[i] = (C)0.5 * (sl[i] + sr[i]);
