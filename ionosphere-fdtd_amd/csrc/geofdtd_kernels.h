@@ -503,11 +503,14 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
         C loss[N], dre[N];
         load_vec<N>(cf, loss);
         load_vec<N>(cf + 4 * pl.coef_q_stride, dre);
-        C t2[N];
+        // the three components are loaded before any is stored (the stores
+        // would otherwise order each load after the previous store)
+        C t2[N], jx[3][N];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) load_vec_stream<N>(jp + q * pl.comp_stride, jx[q]);
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          C x[N];
-          load_vec_stream<N>(jp + q * pl.comp_stride, x);
+          C* x = jx[q];
 #pragma unroll
           for (int i = 0; i < N; ++i)
             x[i] = (j0 + i < pl.band_hi) ? x[i] + (dre[i] * drive[q][i] - loss[i] * x[i]) : x[i];
