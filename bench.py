@@ -27,6 +27,7 @@ sys.path.insert(0, str(REPO))
 sys.path.insert(0, str(REPO / "tests"))
 
 HBM_PEAK_GBS = 8000.0
+C2_RUN_STEPS = 4000  # length of one ensemble run (16 receiver traces per run)
 METRIC = "Mcell-updates/s + % HBM roofline (1 GPU); ensemble trace queries/s at ≤2% err"
 
 
@@ -226,8 +227,10 @@ def ensemble_bench(args) -> dict:
                    "cells": cells,
                    "radial_cells": [len(r.radial_midpoints_m) for r in runs],
                    "graph_chunk": args.chunk, "column_block": shape},
-        "trace_queries_per_s": round(n_traces / elapsed, 2),
+        # one query = one receiver trace of a full 4000-step run
+        "trace_queries_per_s": round(n_traces / (elapsed * C2_RUN_STEPS / args.steps), 2),
         "traces_per_ensemble_run": n_traces,
+        "steps_per_run": C2_RUN_STEPS,
         "roofline": {"bound": "hbm", "kernel": "h_ensemble_kernel+e_ensemble_kernel",
                      "achieved": round(step_bytes / (elapsed / args.steps) / 1e9, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -705,9 +705,6 @@ __global__ void __launch_bounds__(kBlock) h_step_kernel(const HParams<T, C> p) {
 // E half-step
 // ---------------------------------------------------------------------------
 
-// Species-summed J(face, j0 + i) . v for the N lanes of a strip starting at
-// an in-band j0 (lanes past the band read the zero padding), plus the value
-// at j0 - 1 when that layer is in the band.
 // Species sum of one stored component (projected state) for the N lanes of
 // an in-band strip at j0, plus the value at j0 - 1 when that layer is in band.
 template <typename C, int N, bool WANT_PREV>
@@ -732,6 +729,9 @@ __device__ __forceinline__ void plasma_component(const PlasmaTables<C>& pl, int3
   if constexpr (WANT_PREV) *prev = pv;
 }
 
+// Species-summed Cartesian J(face, j0 + i) . v for the N lanes of a strip
+// starting at an in-band j0 (lanes past the band read the zero padding), plus
+// the value at j0 - 1 when that layer is in the band.
 template <typename C, int N, bool WANT_PREV>
 __device__ __forceinline__ void plasma_project(const PlasmaTables<C>& pl, int32_t f, int32_t j0,
                                                const C* v, C* strip, C* prev) {
