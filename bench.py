@@ -290,7 +290,7 @@ def surrogate_bench(args) -> dict:
         "data": "synthetic", "vs_baseline": None,
         "config": {"workload": "2000 held-out C2-style station queries (s5, R=32, 4096 steps, "
                                "512-sample Er traces)", "model": str(model_path.name),
-                   "train": "4096 runs x 8 stations, 6000 Adam steps"},
+                   "train": "8192 runs x 8 stations, 10000 Adam steps"},
         "batched_fdtd_queries_per_s": round(len(queries) / t_fdtd, 2),
         "speedup_vs_batched_fdtd": round(t_fdtd / t_service, 1),
         "served_by_surrogate": int(np.sum(served == "surrogate")),

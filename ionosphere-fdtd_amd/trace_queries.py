@@ -377,9 +377,9 @@ def _main(argv=None) -> int:
     import sys
 
     ap = argparse.ArgumentParser(prog="python -m ionosphere_fdtd_amd.trace_queries")
-    ap.add_argument("--runs", type=int, default=4096)
+    ap.add_argument("--runs", type=int, default=8192)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--steps", type=int, default=6000)
+    ap.add_argument("--steps", type=int, default=10000)
     ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", required=True)
