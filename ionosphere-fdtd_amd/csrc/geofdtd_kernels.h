@@ -397,11 +397,14 @@ __device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, const Scale
 #pragma unroll
     for (int i = 0; i < N; ++i) et[i] = (C)0;
   }
-  const C from_left = __shfl_up(et[N - 1], 1);
-  C prev = ln.left ? from_left : ((k0 > 0) ? load_one(erow + k0 - 1, sc.et_inv) : (C)0);
   C rb[N], ir[N];
   load_vec<N>(p.rd_b + k0, rb);
   load_vec<N>(p.inv_r + k0, ir);
+  // every load is issued before the first wait (the lane shuffle below)
+  C prev_loaded = (C)0;
+  if (!ln.left && k0 > 0) prev_loaded = load_one(erow + k0 - 1, sc.et_inv);
+  const C from_left = __shfl_up(et[N - 1], 1);
+  C prev = ln.left ? from_left : prev_loaded;
   const C h_old0 = ht[0];
   // Rows are padded with zeros and the radial tables are zero beyond R, so
   // the strip is updated unconditionally: Et(R) reads the zero padding (odd
@@ -897,8 +900,6 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
   load_strip(p.hr + (size_t)fr * p.pm + j0, hrr, sc.hr_inv);
   const T* hrow = p.ht + (size_t)e * p.pn;
   load_strip(hrow + j0, ht, sc.ht_inv);
-  const C from_right = __shfl_down(ht[0], 1);
-  ht[N] = ln.right ? from_right : ((j0 + N <= p.R) ? load_one(hrow + j0 + N, sc.ht_inv) : (C)0);
   T* erow = p.et + (size_t)e * p.pm + j0;
   load_strip(erow, et, sc.et_inv);
   C cl[N], cb[N], irb[N], ra[N];
@@ -906,6 +907,10 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
   load_vec<N>(p.cb_et + (size_t)e * p.ca_et_stride + j0, cb);
   load_vec<N>(p.inv_rbar + j0, irb);
   load_vec<N>(p.rh_a + j0, ra);
+  // every load (the plasma reads below included) is issued before the first
+  // wait, the lane shuffle after them
+  C ht_next = (C)0;
+  if (!ln.right && j0 + N <= p.R) ht_next = load_one(hrow + j0 + N, sc.ht_inv);
   C jt[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) jt[i] = (C)0;
@@ -934,6 +939,8 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
       for (int i = 0; i < N; ++i) jt[i] = (C)0.5 * (sl[i] + sr[i]);
     }
   }
+  const C from_right = __shfl_down(ht[0], 1);
+  ht[N] = ln.right ? from_right : ht_next;
   C src[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) src[i] = (C)0;
