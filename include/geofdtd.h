@@ -171,11 +171,12 @@ typedef struct GeoLayout {
   int64_t band_lo;         /* first plasma midpoint layer stored */
   int64_t band_hi;         /* one past the last */
   int64_t plasma_pitch;    /* pitch of plasma rows (>= band_hi - band_lo) */
-  int64_t plasma_components; /* 4 = (J.n, J.t0, J.t1, J.t2) per face (B = 0),
-                                3 = Cartesian J (magnetised) */
+  int64_t plasma_components; /* state values per face, layer and species (3) */
   int64_t vector_elems;    /* elements per 16-byte strip */
   int64_t compute_bytes;   /* 4 (FP32 arithmetic) or 8 (FP64) */
   int64_t si_pitch;        /* row pitch of the surface-impedance memory (16-byte rows) */
+  int64_t plasma_basis;    /* 1 = (J.n, J.t0, J.t1) per face, n the face centre and t_c
+                              its edge tangents (B = 0); 0 = Cartesian J (magnetised) */
 } GeoLayout;
 
 typedef struct GeoDiagOut {

@@ -82,7 +82,7 @@ class GeoLayout(C.Structure):
                 ("band_hi", C.c_int64), ("plasma_pitch", C.c_int64),
                 ("plasma_components", C.c_int64),
                 ("vector_elems", C.c_int64), ("compute_bytes", C.c_int64),
-                ("si_pitch", C.c_int64)]
+                ("si_pitch", C.c_int64), ("plasma_basis", C.c_int64)]
 
 
 class GeoDiagOut(C.Structure):

@@ -632,7 +632,7 @@ struct Plan final : PlanBase {
       band_lo = ph->band_lo / N * N;
       band_hi = ph->band_hi;
       ppitch = round_up(band_hi - band_lo, N);
-      ncomp = magnetized ? 3 : 4;
+      ncomp = 3;
       if (magnetized) {
         recon.assign(ph->plasma_reconstruction, ph->plasma_reconstruction + 9 * F);
       } else {
@@ -641,8 +641,8 @@ struct Plan final : PlanBase {
         // J with E.n = radial average and E.t_c = sum_c' P[c][c'] Et_c',
         // P[c][c'] = t_c . R_c' (R = pinv(T).T, plasma.py:548-559).  The
         // three edge tangents of a face are orthogonal to its centre, hence
-        // coplanar: t_2 = w1 t_0 + w2 t_1, so the update advances J.n, J.t_0
-        // and J.t_1 and writes J.t_2 = w1 J.t_0 + w2 J.t_1 without reading it.
+        // coplanar: t_2 = w1 t_0 + w2 t_1, so the state is J.n, J.t_0, J.t_1
+        // and the scatter forms J.t_2 = w1 J.t_0 + w2 J.t_1.
         if (!ph->face_tangents) return fail(GEO_ERR_INVALID, "face tangents are required");
         recon.assign(8 * F, 0);
         for (int64_t f = 0; f < F; ++f) {
@@ -951,6 +951,7 @@ struct Plan final : PlanBase {
     layout.band_hi = band_hi;
     layout.plasma_pitch = ppitch;
     layout.plasma_components = ncomp;
+    layout.plasma_basis = plasma && !magnetized ? 1 : 0;
     layout.vector_elems = N;
     layout.compute_bytes = sizeof(C);
     layout.si_pitch = si_on ? hp.si_pitch : 0;

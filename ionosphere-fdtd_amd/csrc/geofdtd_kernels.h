@@ -273,9 +273,9 @@ struct PlasmaTables {
   int32_t pitch;                // J row pitch (compute elements)
   int64_t species_stride;       // elements between species blocks (comps * F * pitch)
   int64_t comp_stride;          // elements between component blocks (F * pitch)
-  // B = 0: (species, 4, F, pitch) holding J.n, J.t0, J.t1, J.t2 (the face's
-  // edge tangents are coplanar, so J.t2 = w1 J.t0 + w2 J.t1 is written by the
-  // update and never read by it); magnetised: (species, 3, F, pitch) Cartesian J.
+  // B = 0: (species, 3, F, pitch) holding J.n, J.t0, J.t1 (the face's edge
+  // tangents are coplanar, so J.t2 = w1 J.t0 + w2 J.t1 is formed by the
+  // scatter); magnetised: (species, 3, F, pitch) Cartesian J.
   C* current;
   // B = 0: (F, 8) = P[0][0..2], P[1][0..2] (P[c][c'] = t_c.R_c'), w1, w2;
   // magnetised: (F, 9) R = pinv(T).T
@@ -487,10 +487,10 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
     const int32_t jb0 = j0 - pl.band_lo;
     if constexpr (!MAGNETIZED) {
       // drive projections: E.n = rad, E.t_c = sum_c' P[c][c'] Et_c' (c = 0, 1)
-      C rc[8];
+      C rc[6];
       load_vec<4>(pl.recon + 8 * f, rc);
-      load_vec<4>(pl.recon + 8 * f + 4, rc + 4);
-      const C w1 = rc[6], w2 = rc[7];
+      rc[4] = pl.recon[8 * f + 4];
+      rc[5] = pl.recon[8 * f + 5];
       C drive[3][N];
 #pragma unroll
       for (int i = 0; i < N; ++i) {
@@ -508,7 +508,7 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
         load_vec<N>(cf + 4 * pl.coef_q_stride, dre);
         // the three components are loaded before any is stored (the stores
         // would otherwise order each load after the previous store)
-        C t2[N], jx[3][N];
+        C jx[3][N];
 #pragma unroll
         for (int q = 0; q < 3; ++q) load_vec_stream<N>(jp + q * pl.comp_stride, jx[q]);
 #pragma unroll
@@ -518,15 +518,7 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
           for (int i = 0; i < N; ++i)
             x[i] = (j0 + i < pl.band_hi) ? x[i] + (dre[i] * drive[q][i] - loss[i] * x[i]) : x[i];
           store_vec_stream<N>(jp + q * pl.comp_stride, x);
-          if (q == 1) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) t2[i] = w1 * x[i];
-          } else if (q == 2) {
-#pragma unroll
-            for (int i = 0; i < N; ++i) t2[i] += w2 * x[i];
-          }
         }
-        store_vec_stream<N>(jp + 3 * pl.comp_stride, t2);
       }
     } else {
       C rc[9];
@@ -925,14 +917,40 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
         plasma_project<C, N, false>(pl, fl, j0, tl, sl, &unused);
         plasma_project<C, N, false>(pl, fr, j0, tr, sr, &unused);
       } else {
+        // J.t of the edge's slot in each face: slots 0 and 1 are stored,
+        // slot 2 is w1 J.t0 + w2 J.t1 (J.t0 read in place of the stored
+        // value, J.t1 read in addition); weights (1, 0) leave a stored
+        // slot's value unchanged
         const int32_t slot_l = pl.edge_slots[2 * e], slot_r = pl.edge_slots[2 * e + 1];
-        if (pl.n_species == 1) {  // both sides loaded together
+        const int32_t ql = slot_l < 2 ? 1 + slot_l : 1, qr = slot_r < 2 ? 1 + slot_r : 1;
+        C xl[N], xr[N];
+        C wl1 = (C)1, wl2 = (C)0, wr1 = (C)1, wr2 = (C)0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) xl[i] = xr[i] = (C)0;
+        if (slot_l == 2) {
+          wl1 = pl.recon[8 * fl + 6];
+          wl2 = pl.recon[8 * fl + 7];
+        }
+        if (slot_r == 2) {
+          wr1 = pl.recon[8 * fr + 6];
+          wr2 = pl.recon[8 * fr + 7];
+        }
+        if (pl.n_species == 1) {  // every side's loads issued together
           const int32_t jb0 = j0 - pl.band_lo;
-          load_vec<N>(pl.current + (1 + slot_l) * pl.comp_stride + (size_t)fl * pl.pitch + jb0, sl);
-          load_vec<N>(pl.current + (1 + slot_r) * pl.comp_stride + (size_t)fr * pl.pitch + jb0, sr);
+          load_vec<N>(pl.current + ql * pl.comp_stride + (size_t)fl * pl.pitch + jb0, sl);
+          load_vec<N>(pl.current + qr * pl.comp_stride + (size_t)fr * pl.pitch + jb0, sr);
+          if (slot_l == 2) load_vec<N>(pl.current + 2 * pl.comp_stride + (size_t)fl * pl.pitch + jb0, xl);
+          if (slot_r == 2) load_vec<N>(pl.current + 2 * pl.comp_stride + (size_t)fr * pl.pitch + jb0, xr);
         } else {
-          plasma_component<C, N, false>(pl, fl, 1 + slot_l, j0, sl, &unused);
-          plasma_component<C, N, false>(pl, fr, 1 + slot_r, j0, sr, &unused);
+          plasma_component<C, N, false>(pl, fl, ql, j0, sl, &unused);
+          plasma_component<C, N, false>(pl, fr, qr, j0, sr, &unused);
+          if (slot_l == 2) plasma_component<C, N, false>(pl, fl, 2, j0, xl, &unused);
+          if (slot_r == 2) plasma_component<C, N, false>(pl, fr, 2, j0, xr, &unused);
+        }
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          sl[i] = wl1 * sl[i] + wl2 * xl[i];
+          sr[i] = wr1 * sr[i] + wr2 * xr[i];
         }
       }
 #pragma unroll

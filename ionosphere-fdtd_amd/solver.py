@@ -543,22 +543,30 @@ class GeodesicFDTD:
                   lambda self, v: self._set_field("ht", v))
 
     def _plasma_bases(self):
-        """Device (normal, tangents, reconstruction) for state <-> Cartesian J."""
+        """Device (normal, tangents, reconstruction, w) for state <-> Cartesian J,
+        with t_2 = w[:, 0] t_0 + w[:, 1] t_1 (a face's edge tangents are
+        orthogonal to its centre, hence coplanar)."""
         if getattr(self, "_plasma_basis_cache", None) is None:
             pt = self._plasma_info
             cdt = self.backend.compute_dtype
+            t = pt["face_tangents"]
+            gram = np.einsum("fcd,fed->fce", t[:, :2], t[:, :2])
+            rhs = np.einsum("fcd,fd->fc", t[:, :2], t[:, 2])
+            w = np.linalg.solve(gram, rhs[..., None])[..., 0]
             self._plasma_basis_cache = (self._tensor(self.mesh.face_centers, cdt),
-                                        self._tensor(pt["face_tangents"], cdt),
-                                        self._tensor(pt["reconstruction"], cdt))
+                                        self._tensor(t, cdt),
+                                        self._tensor(pt["reconstruction"], cdt),
+                                        self._tensor(w, cdt))
         return self._plasma_basis_cache
 
     @property
     def plasma_current_density(self) -> list:
         """Per-species (face, radial, 3) Cartesian current density (copies).
 
-        With B = 0 the device state holds the projections (J.n, J.t_c) the
-        scatter needs; J = (J.n) n + sum_c (J.t_c) R_c recovers the Cartesian
-        vector exactly (R = pinv(T).T spans the tangent plane)."""
+        With B = 0 the device state holds the projections (J.n, J.t_0, J.t_1)
+        the scatter needs; with J.t_2 = w1 J.t_0 + w2 J.t_1,
+        J = (J.n) n + sum_c (J.t_c) R_c recovers the Cartesian vector exactly
+        (R = pinv(T).T spans the tangent plane)."""
         torch = self.backend.torch
         R = len(self.radial_midpoints_m)
         if self._plasma_buf is None:
@@ -570,12 +578,14 @@ class GeodesicFDTD:
         out = []
         for s in range(self._plasma_buf.shape[0]):
             state = self._plasma_buf[s, :, :, : hi - lo]  # (comps, F, band)
-            if self._layout.plasma_components == 3:
+            if self._layout.plasma_basis == 0:
                 band_j = state.permute(1, 2, 0)
             else:
-                normal, _, recon = self._plasma_bases()
+                normal, _, recon, w = self._plasma_bases()
+                t2 = w[:, 0, None] * state[1] + w[:, 1, None] * state[2]
+                tang = torch.stack((state[1], state[2], t2))
                 band_j = (state[0][..., None] * normal[:, None, :]
-                          + torch.einsum("cfb,fcd->fbd", state[1:], recon))
+                          + torch.einsum("cfb,fcd->fbd", tang, recon))
             full = torch.zeros((self.mesh.n_faces, R, 3), dtype=self._plasma_buf.dtype,
                                device=self._plasma_buf.device)
             full[:, lo:hi, :] = band_j
@@ -596,13 +606,13 @@ class GeodesicFDTD:
         if outside:
             raise ValueError("plasma current is nonzero outside the layers with plasma drive")
         band = data[:, lo:hi, :]
-        if self._layout.plasma_components == 3:
+        if self._layout.plasma_basis == 0:
             self._plasma_buf[index, :, :, : hi - lo] = band.permute(2, 0, 1)
         else:
-            normal, tangents, _ = self._plasma_bases()
+            normal, tangents, _, _ = self._plasma_bases()
             self._plasma_buf[index, 0, :, : hi - lo] = (band * normal[:, None, :]).sum(-1)
             self._plasma_buf[index, 1:, :, : hi - lo] = self.backend.torch.einsum(
-                "fbd,fcd->cfb", band, tangents)
+                "fbd,fcd->cfb", band, tangents[:, :2])
 
     # --------------------------------------------------------------- stepping
     def _stream(self):

@@ -231,6 +231,43 @@ def test_fp32_plasma_within_tolerance_of_reference(name):
         assert err <= FP32_BAR, k
 
 
+@pytest.mark.parametrize("dtype,tol", [("float64", 1e-13), ("float32", 2e-6)])
+def test_drude_state_is_the_three_independent_projections(dtype, tol):
+    """B = 0 device state per face: J.n, J.t0, J.t1.  A face's edge tangents
+    are orthogonal to its centre, hence coplanar (t2 = w1 t0 + w2 t1), so
+    J.t2 is not stored.  After 50 driven steps the Cartesian current read back
+    must project onto n, t0 and t1 as the stored components and onto t2 as
+    w1 J.t0 + w2 J.t1 (w from an independent FP64 least-squares fit), and
+    writing that current back must reproduce the state."""
+    sc = S.scenario("plasma_random")
+    sim = S.build_sim(sc, dtype=dtype, compile_step=True)
+    S.apply_seed_fields(sim, lambda k, v: setattr(sim, k, v))
+    sim.step(50)
+    lo, hi = sim._layout.band_lo, sim._layout.band_hi
+    assert sim._layout.plasma_basis == 1
+    state = sim._plasma_buf[..., : hi - lo].double().cpu().numpy()  # (species, 3, F, band)
+    assert state.shape[1] == 3
+    t = np.asarray(sim._plasma_info["face_tangents"], dtype=np.float64)  # (F, 3 slots, 3)
+    n = np.asarray(sim.mesh.face_centers, dtype=np.float64)
+    assert np.abs(np.einsum("fcd,fd->fc", t, n)).max() < 1e-14  # tangents orthogonal to n
+    w = np.stack([np.linalg.lstsq(t[f, :2].T, t[f, 2], rcond=None)[0] for f in range(len(t))])
+    assert np.abs(np.einsum("fc,fcd->fd", w, t[:, :2]) - t[:, 2]).max() < 1e-12  # coplanar
+    scale = float(np.abs(state).max())
+    assert scale > 0
+    currents = sim.plasma_current_density
+    for s, J in enumerate(currents):
+        J = J.double().cpu().numpy()[:, lo:hi]  # (F, band, 3)
+        want = [state[s, 0], state[s, 1], state[s, 2],
+                w[:, 0, None] * state[s, 1] + w[:, 1, None] * state[s, 2]]
+        for vec, expect in zip((n, t[:, 0], t[:, 1], t[:, 2]), want):
+            np.testing.assert_allclose(np.einsum("fbd,fd->fb", J, vec), expect, rtol=0,
+                                       atol=10 * tol * scale)
+    for s, J in enumerate(currents):
+        sim.set_plasma_current_density(s, J)
+    again = sim._plasma_buf[..., : hi - lo].double().cpu().numpy()
+    np.testing.assert_allclose(again, state, rtol=0, atol=10 * tol * scale)
+
+
 @pytest.mark.parametrize("dtype,bar", [("float64", 1e-13), ("float32", 1e-5)])
 def test_discrete_div_b_stays_at_roundoff_on_device(dtype, bar):
     """Derived invariant: the H kernel conserves the prism-cell divergence of B;
