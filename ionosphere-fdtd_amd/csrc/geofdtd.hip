@@ -622,7 +622,7 @@ struct Plan final : PlanBase {
     plasma = ph->n_species > 0;
     magnetized = plasma && ph->magnetized != 0;
     int64_t band_lo = 0, band_hi = 0, ppitch = 0;
-    std::vector<C> recon, center, ltan, rtan, vfw, coef, bdir;
+    std::vector<C> recon, center, ltan, rtan, vfw, coef, bdir, sidew;
     std::vector<int32_t> vfaces, eslots;
     int64_t ncomp = 0;
     if (plasma) {
@@ -680,6 +680,14 @@ struct Plan final : PlanBase {
             if (slot < 0) return fail(GEO_ERR_INVALID, "edge is absent from an adjacent face");
             eslots[2 * e + side] = slot;
           }
+        sidew.assign(4 * E, 0);
+        for (int64_t e = 0; e < E; ++e)
+          for (int side = 0; side < 2; ++side) {
+            const int64_t f = side == 0 ? m->edge_left_faces[e] : m->edge_right_faces[e];
+            const bool dep = eslots[2 * e + side] == 2;
+            sidew[4 * e + 2 * side] = dep ? recon[8 * f + 6] : (C)1;
+            sidew[4 * e + 2 * side + 1] = dep ? recon[8 * f + 7] : (C)0;
+          }
       }
       center.assign(4 * F, 0);
       for (int64_t f = 0; f < F; ++f)
@@ -734,6 +742,7 @@ struct Plan final : PlanBase {
     C *d_recon = nullptr, *d_center = nullptr, *d_lt = nullptr, *d_rt = nullptr;
     C *d_vfw = nullptr, *d_coef = nullptr, *d_b = nullptr;
     int32_t* d_es = nullptr;
+    C* d_sw = nullptr;
     UP(edges, d_edges); UP(edge_lr, d_lr); UP(face_edges, d_fe); UP(faces, d_faces);
     UP(vedges, d_ve); UP(vorder, d_vo); UP(edge_grad, d_eg); UP(edge_dual, d_ed); UP(face_curl, d_fc);
     UP(vw, d_vw); UP(inv_r, d_ir); UP(rd_b, d_rdb); UP(inv_rbar, d_irb);
@@ -747,6 +756,7 @@ struct Plan final : PlanBase {
         UP(bdir, d_b);
       } else {
         UP(eslots, d_es);
+        UP(sidew, d_sw);
       }
     }
 #undef UP
@@ -778,6 +788,7 @@ struct Plan final : PlanBase {
       pl.comp_stride = F * ppitch;
       pl.species_stride = ncomp * F * ppitch;
       pl.edge_slots = d_es;
+      pl.side_w = d_sw;
       pl.recon = d_recon;
       pl.center = d_center;
       pl.left_tan = d_lt;

@@ -281,6 +281,7 @@ struct PlasmaTables {
   // magnetised: (F, 9) R = pinv(T).T
   const C* recon;
   const int32_t* edge_slots;    // (E, 2) slot of the edge in its left / right face
+  const C* side_w;              // (E, 4) B = 0: (w1, w2) left side, then right; (1, 0) for stored slots
   const C* center;              // (F, 4) unit normal, padded
   const C* left_tan;            // (E, 4) padded
   const C* right_tan;           // (E, 4) padded
@@ -924,17 +925,11 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
         const int32_t slot_l = pl.edge_slots[2 * e], slot_r = pl.edge_slots[2 * e + 1];
         const int32_t ql = slot_l < 2 ? 1 + slot_l : 1, qr = slot_r < 2 ? 1 + slot_r : 1;
         C xl[N], xr[N];
-        C wl1 = (C)1, wl2 = (C)0, wr1 = (C)1, wr2 = (C)0;
+        C sw[4];
+        load_vec<4>(pl.side_w + 4 * e, sw);
+        const C wl1 = sw[0], wl2 = sw[1], wr1 = sw[2], wr2 = sw[3];
 #pragma unroll
         for (int i = 0; i < N; ++i) xl[i] = xr[i] = (C)0;
-        if (slot_l == 2) {
-          wl1 = pl.recon[8 * fl + 6];
-          wl2 = pl.recon[8 * fl + 7];
-        }
-        if (slot_r == 2) {
-          wr1 = pl.recon[8 * fr + 6];
-          wr2 = pl.recon[8 * fr + 7];
-        }
         if (pl.n_species == 1) {  // every side's loads issued together
           const int32_t jb0 = j0 - pl.band_lo;
           load_vec<N>(pl.current + ql * pl.comp_stride + (size_t)fl * pl.pitch + jb0, sl);
