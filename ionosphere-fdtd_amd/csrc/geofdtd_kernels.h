@@ -779,12 +779,6 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Sca
 #pragma unroll
     for (int i = 0; i < N; ++i) curl[i] += vw[s] * h[i];
   }
-  T* erow = p.er + (size_t)v * p.pn + k0;
-  C er[N], cl[N], cb[N], ir[N];
-  load_strip(erow, er, sc.er_inv);
-  load_vec<N>(p.cl_er + (size_t)v * p.ca_er_stride + k0, cl);
-  load_vec<N>(p.cb_er + (size_t)v * p.ca_er_stride + k0, cb);
-  load_vec<N>(p.inv_r + k0, ir);
   C jr[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) jr[i] = (C)0;
@@ -801,19 +795,25 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Sca
       load_vec<4>(pl.vertex_faces + 8 * v + 4, vf + 4);
       load_vec<8>(pl.vertex_face_w + 8 * v, vfw);
       if (!MAGNETIZED && pl.n_species == 1) {
-        // One species: the six face strips are loaded together (padding
-        // slots name face 0 with weight 0), and the weighted sum at midpoint
+        // One species: the six face strips are loaded three at a time
+        // (padding slots name face 0 with weight 0; the scheduling barrier
+        // keeps the second three from being hoisted, which holds the kernel
+        // at 72 VGPRs and 7 waves per SIMD), and the weighted sum at midpoint
         // k0 - 1 is the left lane's last sum (same vertex, same weights, same
         // order of additions).
         const int32_t jb0 = k0 - pl.band_lo;
         if (k0 >= pl.band_lo && k0 < pl.band_hi) {
-          C x[6][N];
 #pragma unroll
-          for (int s = 0; s < 6; ++s) load_vec<N>(pl.current + (size_t)vf[s] * pl.pitch + jb0, x[s]);
+          for (int g = 0; g < 6; g += 3) {
+            C x[3][N];
 #pragma unroll
-          for (int s = 0; s < 6; ++s)
+            for (int s = 0; s < 3; ++s) load_vec<N>(pl.current + (size_t)vf[g + s] * pl.pitch + jb0, x[s]);
 #pragma unroll
-            for (int i = 0; i < N; ++i) rv[i + 1] += x[s][i] * vfw[s];
+            for (int s = 0; s < 3; ++s)
+#pragma unroll
+              for (int i = 0; i < N; ++i) rv[i + 1] += x[s][i] * vfw[g + s];
+            __builtin_amdgcn_sched_barrier(0);
+          }
         }
         const C from_left = __shfl_up(rv[N], 1);
         const bool has_prev = k0 - 1 >= pl.band_lo && k0 - 1 < pl.band_hi;
@@ -857,6 +857,15 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Sca
       }
     }
   }
+  // The own row and its coefficients are loaded only now: live across the
+  // plasma sums they held the kernel at 96 VGPRs and 5 waves per SIMD
+  // against 80 and 6 (before the three-at-a-time grouping of those sums).
+  T* erow = p.er + (size_t)v * p.pn + k0;
+  C er[N], cl[N], cb[N], ir[N];
+  load_strip(erow, er, sc.er_inv);
+  load_vec<N>(p.cl_er + (size_t)v * p.ca_er_stride + k0, cl);
+  load_vec<N>(p.cb_er + (size_t)v * p.ca_er_stride + k0, cb);
+  load_vec<N>(p.inv_r + k0, ir);
   C src[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) src[i] = (C)0;
