@@ -91,6 +91,11 @@ def scenario(name: str, **overrides) -> dict:
         # seeded fields everywhere so the plasma band is driven from the first step
         base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3}
         base["seed_fields"] = True
+    elif name == "drude_two_species":
+        # B = 0 with electrons and O+ ions: the general species loop of the
+        # Drude update and scatter, driven by seeded fields from the first step
+        base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3, "ions": True}
+        base["seed_fields"] = True
     elif name == "magnetized_random":
         base.update(subdivision=1, radial_cells=12, seed_fields=True)
         base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3, "magnetized": True}
@@ -231,6 +236,7 @@ def _plasma_arrays(sc, mesh, midpoint_altitudes):
         tilt = np.stack((n[:, 1], -n[:, 0], 0.3 * np.ones(F)), axis=1)
         b = np.broadcast_to((5e-5 * tilt / np.linalg.norm(tilt, axis=1, keepdims=True))[:, None, :],
                             (F, R, 3)).copy()
+    if p.get("magnetized") or p.get("ions"):
         species.append(("o_plus", 1.602_176_634e-19, 16 * 1.672_621_923_69e-27, 0.5 * dens,
                         0.01 * coll))
     return b, species

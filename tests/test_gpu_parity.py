@@ -66,6 +66,29 @@ def test_fp64_fields_and_traces_match_oracle(name, steps):
         assert S.rel_l2(got_j, want_j) <= FP64_BAR
 
 
+@pytest.mark.parametrize("dtype,bar", [("float64", FP64_BAR), ("float32", FP32_BAR)])
+def test_two_species_drude_matches_oracle(dtype, bar):
+    """B = 0 with two species (electrons and O+): the general species loop of
+    the plasma update and of both scatters, including the J.t2 combination,
+    which the single-species paths bypass.  Seeded fields drive the band from
+    the first step."""
+    sc = S.scenario("drude_two_species")
+    sim = S.build_sim(sc, dtype=dtype, compile_step=True)
+    ref = S.build_oracle(sc)
+    assert len(sim.plasma.species) == 2
+    _seed_fields(sim, ref)
+    sim.step(120)
+    ref.step(120)
+    _compare(sim, ref, bar, f"drude_two_species {dtype}")
+    for s in range(2):
+        got_j = sim.to_numpy(sim.plasma_current_density[s]).astype(np.float64)
+        want_j = ref.coupler.current_density[s]
+        assert float(np.max(np.abs(want_j))) > 0
+        err = S.rel_l2(got_j, want_j)
+        print("species", s, "J rel-L2", f"{err:.2e}", flush=True)
+        assert err <= bar
+
+
 def test_fp64_random_fields_pec_vacuum():
     """The reference benchmark workload: vacuum, PEC, random fields x 1e-6."""
     sc = S.scenario("pec_vacuum")
