@@ -80,7 +80,9 @@ class GeodesicMesh:
         return value
 
     def edge_midpoints(self) -> FloatArray:
-        return _normalize(self.vertices[self.edges].sum(axis=1))
+        """Unit edge midpoints (read-only, computed once per mesh)."""
+        return _mesh_derived(self, "edge_midpoints",
+                             lambda: _normalize(self.vertices[self.edges].sum(axis=1)))
 
     def edge_diamond_solid_angles(self) -> FloatArray:
         """Disjoint primal-face area associated with each edge (mesh.py:93-104)."""
@@ -394,11 +396,37 @@ def _optimize_edge_lengths(vertices: FloatArray, faces: IntArray, steps: int) ->
     return result
 
 
+_DERIVED: list = []
+
+
+def _mesh_derived(mesh, name: str, compute):
+    """Mesh-only arrays computed once and shared (read-only) by every run on
+    the mesh; the last few meshes are kept."""
+    for cached, values in _DERIVED:
+        if cached is mesh:
+            break
+    else:
+        values = {}
+        _DERIVED.insert(0, (mesh, values))
+        del _DERIVED[4:]
+    if name not in values:
+        result = compute()
+        for array in (result if isinstance(result, tuple) else (result,)):
+            array.setflags(write=False)
+        values[name] = result
+    return values[name]
+
+
 def vertex_incidence(mesh: GeodesicMesh) -> tuple[IntArray, IntArray]:
     """Padded dual-cell incidence: per vertex its edges in edge order, sign +1
     where the vertex is the tail and -1 where it is the head; degree-5 rows
     padded with (edge 0, sign 0).  Same table as the reference's
-    numpy_backend.py:46-76 (stable argsort of the concatenated endpoints)."""
+    numpy_backend.py:46-76 (stable argsort of the concatenated endpoints).
+    Read-only, computed once per mesh."""
+    return _mesh_derived(mesh, "vertex_incidence", lambda: _vertex_incidence(mesh))
+
+
+def _vertex_incidence(mesh: GeodesicMesh) -> tuple[IntArray, IntArray]:
     E = mesh.n_edges
     verts = np.concatenate((mesh.edges[:, 0], mesh.edges[:, 1]))
     incident = np.concatenate((np.arange(E), np.arange(E)))

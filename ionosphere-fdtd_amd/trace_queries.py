@@ -188,6 +188,23 @@ def build_run(mesh, h_prime_km, beta_per_km, log10_sigma, src_lat, src_lon, carr
                         compile_step=True, compile_chunk_size=128)
 
 
+_RECEIVER_TABLES: dict = {}
+
+
+def _receiver_table(mesh, altitudes, lat: float, lon: float):
+    """Surface receiver distribution at (lat, lon), kept per mesh: the station
+    network repeats the same few points in every query batch."""
+    key = (id(mesh), float(lat), float(lon))
+    hit = _RECEIVER_TABLES.get(key)
+    if hit is not None and hit[0] is mesh:
+        return hit[1]
+    table = barycentric_distribution(mesh, altitudes, lat, lon, 0.0)
+    _RECEIVER_TABLES[key] = (mesh, table)
+    if len(_RECEIVER_TABLES) > 4096:
+        _RECEIVER_TABLES.pop(next(iter(_RECEIVER_TABLES)))
+    return table
+
+
 def fdtd_traces(queries: np.ndarray, mesh=None, *, batch: int = 64, workers: int = 8,
                 progress=None) -> np.ndarray:
     """Answer queries with FDTD: one run per distinct configuration, all of its
@@ -212,7 +229,7 @@ def fdtd_traces(queries: np.ndarray, mesh=None, *, batch: int = 64, workers: int
         idx = keys[key]
         pts = [(q[i, 5], q[i, 6]) for i in idx]
         pts += [pts[0]] * (width - len(pts))  # equal receiver counts across the ensemble
-        tables = [barycentric_distribution(mesh, alt, lat, lon, 0.0) for lat, lon in pts]
+        tables = [_receiver_table(mesh, alt, lat, lon) for lat, lon in pts]
         rx = (np.asarray([t[0] for t in tables]), np.asarray([t[1] for t in tables]),
               np.asarray([t[2] for t in tables]))
         return sim, rx
@@ -225,7 +242,7 @@ def fdtd_traces(queries: np.ndarray, mesh=None, *, batch: int = 64, workers: int
     torch.cuda.init()
     if torch.cuda.device_count() < 1:
         raise RuntimeError("no ROCm device is visible")
-    with ThreadPoolExecutor(max_workers=workers) as pool:
+    with ThreadPoolExecutor(max_workers=max(1, min(workers, len(order)))) as pool:
         for start in range(0, len(order), batch):
             group = order[start:start + batch]
             built = list(pool.map(make, group))
