@@ -689,16 +689,20 @@ struct Plan final : PlanBase {
             sidew[4 * e + 2 * side + 1] = dep ? recon[8 * f + 7] : (C)0;
           }
       }
-      center.assign(4 * F, 0);
-      for (int64_t f = 0; f < F; ++f)
-        for (int d = 0; d < 3; ++d) center[4 * f + d] = (C)ph->face_centers[3 * f + d];
-      ltan.assign(4 * E, 0);
-      rtan.assign(4 * E, 0);
-      for (int64_t e = 0; e < E; ++e)
-        for (int d = 0; d < 3; ++d) {
-          ltan[4 * e + d] = (C)ph->left_tangents[3 * e + d];
-          rtan[4 * e + d] = (C)ph->right_tangents[3 * e + d];
-        }
+      if (magnetized) {  // Cartesian state: normals and edge tangents on the device
+        if (!ph->face_centers || !ph->left_tangents || !ph->right_tangents)
+          return fail(GEO_ERR_INVALID, "face centres and edge tangents are required");
+        center.assign(4 * F, 0);
+        for (int64_t f = 0; f < F; ++f)
+          for (int d = 0; d < 3; ++d) center[4 * f + d] = (C)ph->face_centers[3 * f + d];
+        ltan.assign(4 * E, 0);
+        rtan.assign(4 * E, 0);
+        for (int64_t e = 0; e < E; ++e)
+          for (int d = 0; d < 3; ++d) {
+            ltan[4 * e + d] = (C)ph->left_tangents[3 * e + d];
+            rtan[4 * e + d] = (C)ph->right_tangents[3 * e + d];
+          }
+      }
       if (ph->vf_width < 1 || ph->vf_width > 6)
         return fail(GEO_ERR_INVALID, "vertex-face width must be at most 6");
       vfaces.assign(8 * V, 0);
@@ -750,9 +754,10 @@ struct Plan final : PlanBase {
     UP(cl_et, d_caet); UP(cb_et, d_cbet);
     if (si_on) UP(si_scale, d_sis);
     if (plasma) {
-      UP(recon, d_recon); UP(center, d_center); UP(ltan, d_lt); UP(rtan, d_rt);
+      UP(recon, d_recon);
       UP(vfaces, d_vf); UP(vfw, d_vfw); UP(coef, d_coef);
       if (magnetized) {
+        UP(center, d_center); UP(ltan, d_lt); UP(rtan, d_rt);
         UP(bdir, d_b);
       } else {
         UP(eslots, d_es);

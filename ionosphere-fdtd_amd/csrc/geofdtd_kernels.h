@@ -123,13 +123,6 @@ __device__ __forceinline__ C load_one(const T* __restrict__ p, C inv_scale) {
   else return to_c<C>(*p);
 }
 
-template <typename T, typename C>
-__device__ __forceinline__ void store_one(T* __restrict__ p, C x, C scale) {
-  if constexpr (is_narrow<T>()) *p = from_c<T>(x * scale);
-  else *p = from_c<T>(x);
-}
-
-
 // Storage scales of the four fields as seen by one kernel launch: read once
 // from the device block for narrow types, identically 1 otherwise.
 template <typename C>
