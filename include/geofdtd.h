@@ -31,8 +31,10 @@
  *   geo_set_receivers <- record_er_observations argument upload
  *                        (solver.py:1097-1120)
  *   geo_diagnostics   <- diagnostics() max_abs (solver.py:1027-1030) and the
- *                        FP64 energy reductions of
- *                        verification/physics_diagnostics/model.py:242-258
+ *                        FP64 energy / conductive-loss reductions of
+ *                        verification/physics_diagnostics/model.py:242-267
+ *   geo_radial_profiles <- the per-layer energy and loss profiles of the same
+ *                        sampler (model.py:136-186, 242-267)
  *   geo_last_error    <- the Python exceptions the reference raises
  *
  * Error convention: 0 = success; GEO_ERR_INVALID maps to ValueError,
@@ -41,7 +43,8 @@
  * calling thread's last failure.
  *
  * Threading: one host thread per plan.  All device work is enqueued on the
- * caller's stream; no call synchronises the host except geo_diagnostics.
+ * caller's stream; no call synchronises the host except geo_diagnostics and
+ * geo_radial_profiles.
  */
 #ifndef GEOFDTD_H
 #define GEOFDTD_H
@@ -150,6 +153,9 @@ typedef struct GeoPhysicsDesc {
   int64_t plasma_coef_rows;
   const double* plasma_b;            /* (plasma_b_rows, R, 3) unit B */
   int64_t plasma_b_rows;             /* 1 or F */
+  /* Conductivity (S/m) with the same rows as the coefficients, for the
+   * conductive-loss diagnostic; NULL means no loss. */
+  const double* sigma_er; const double* sigma_et;
 } GeoPhysicsDesc;
 
 /* Device buffers owned by the caller.  Pitches count elements. */
@@ -190,7 +196,18 @@ typedef struct GeoDiagOut {
   double divb_max_abs;     /* max |D| */
   double divb_flux_scale;  /* max over cells of the summed |flux| terms */
   int64_t storage_overflow; /* 1 once a rescale point saw a non-finite stored value */
+  double loss_er, loss_et;  /* conductive loss sigma*E^2*w (W) */
+  double sumsq_er, sumsq_et, sumsq_hr, sumsq_ht; /* sum of squares of the finite
+                                                     (entity, layer) values, for RMS */
+  int64_t nonfinite_er, nonfinite_et, nonfinite_hr, nonfinite_ht; /* per field */
 } GeoDiagOut;
+
+/* Host arrays for geo_radial_profiles; a NULL pointer skips that profile.
+ * Lengths: er / ht R+1 nodes, et / hr R midpoints. */
+typedef struct GeoProfiles {
+  double* energy_er; double* energy_et; double* energy_hr; double* energy_ht; /* J per layer */
+  double* loss_er; double* loss_et;                                           /* W per layer */
+} GeoProfiles;
 
 typedef struct GeoPlan GeoPlan;
 
@@ -213,6 +230,11 @@ int geo_set_receivers(GeoPlan* plan, const int64_t* vertices, const int64_t* lay
 int geo_sample_er(GeoPlan* plan, void* row, void* stream);
 /* FP64 reductions over the bound state; synchronises the stream. */
 int geo_diagnostics(GeoPlan* plan, GeoDiagOut* out, void* stream);
+/* Per-layer FP64 energy and conductive-loss profiles (the horizontal sums of
+ * verification/physics_diagnostics/model.py:242-267 times the radial
+ * weights r^2 L_k at nodes, rbar^2 dr at midpoints); synchronises the
+ * stream. */
+int geo_radial_profiles(GeoPlan* plan, const GeoProfiles* out, void* stream);
 /* Average kernel durations (ms) of n_steps individually launched and
  * event-timed steps: out_ms[0] = H kernel, out_ms[1] = E kernel. */
 int geo_time_kernels(GeoPlan* plan, int64_t n_steps, void* stream, double* out_ms);

@@ -13,6 +13,7 @@ from .constants import C_0, EARTH_RADIUS_M, EPSILON_0, MU_0
 from .ensemble import EnsembleFDTD
 from .materials import EarthIonosphereMaterial
 from .mesh import GeodesicMesh, build_geodesic_mesh, build_geodesic_mesh_from_topology
+from .physics_diagnostics import PhysicsDiagnosticSampler, PhysicsSnapshot
 from .plasma import ColdPlasmaSpecies, MeshPlasmaModel, electron_drude_model
 from .radial_grid import (RadialGridValidation, RadialRefinementRegion,
                           build_refined_radial_grid, validate_radial_grid)
@@ -25,6 +26,7 @@ __all__ = [
     "ConductiveHalfSpaceSurface", "EARTH_RADIUS_M", "EPSILON_0", "EarthIonosphereMaterial",
     "EnsembleFDTD",
     "GaussianCurrent", "GeodesicFDTD", "GeodesicMesh", "MU_0", "MeshPlasmaModel",
+    "PhysicsDiagnosticSampler", "PhysicsSnapshot",
     "RadialGridValidation", "RadialRefinementRegion", "SimulationConfig",
     "TangentialGaussianCurrent", "build_geodesic_mesh", "build_geodesic_mesh_from_topology",
     "build_refined_radial_grid", "create_backend", "electron_drude_model",

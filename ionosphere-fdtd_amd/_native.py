@@ -66,6 +66,7 @@ class GeoPhysicsDesc(C.Structure):
         ("vertex_faces", _p_i64), ("vertex_face_weights", _p_f64), ("vf_width", C.c_int64),
         ("plasma_coef", _p_f64), ("plasma_coef_rows", C.c_int64),
         ("plasma_b", _p_f64), ("plasma_b_rows", C.c_int64),
+        ("sigma_er", _p_f64), ("sigma_et", _p_f64),
     ]
 
 
@@ -92,12 +93,23 @@ class GeoDiagOut(C.Structure):
                 ("energy_hr", C.c_double), ("energy_ht", C.c_double),
                 ("nonfinite", C.c_int64),
                 ("divb_max_abs", C.c_double), ("divb_flux_scale", C.c_double),
-                ("storage_overflow", C.c_int64)]
+                ("storage_overflow", C.c_int64),
+                ("loss_er", C.c_double), ("loss_et", C.c_double),
+                ("sumsq_er", C.c_double), ("sumsq_et", C.c_double),
+                ("sumsq_hr", C.c_double), ("sumsq_ht", C.c_double),
+                ("nonfinite_er", C.c_int64), ("nonfinite_et", C.c_int64),
+                ("nonfinite_hr", C.c_int64), ("nonfinite_ht", C.c_int64)]
+
+
+class GeoProfiles(C.Structure):
+    _fields_ = [("energy_er", _p_f64), ("energy_et", _p_f64), ("energy_hr", _p_f64),
+                ("energy_ht", _p_f64), ("loss_er", _p_f64), ("loss_et", _p_f64)]
 
 
 EXPORTED_SYMBOLS = (
     "geo_plan_create", "geo_plan_layout", "geo_plan_bind", "geo_step", "geo_set_receivers",
-    "geo_sample_er", "geo_diagnostics", "geo_time_kernels", "geo_plan_destroy",
+    "geo_sample_er", "geo_diagnostics", "geo_radial_profiles", "geo_time_kernels",
+    "geo_plan_destroy",
     "geo_last_error", "geo_version", "geo_stream_probe", "geo_get_storage_scales",
     "geo_set_storage_scales", "geo_ensemble_create", "geo_ensemble_configure",
     "geo_ensemble_tune", "geo_ensemble_step", "geo_ensemble_destroy",
@@ -128,6 +140,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.geo_set_receivers.argtypes = [vp, _p_i64, _p_i64, _p_f64, C.c_int32]
     lib.geo_sample_er.argtypes = [vp, vp, vp]
     lib.geo_diagnostics.argtypes = [vp, C.POINTER(GeoDiagOut), vp]
+    lib.geo_radial_profiles.argtypes = [vp, C.POINTER(GeoProfiles), vp]
     lib.geo_time_kernels.argtypes = [vp, C.c_int64, vp, _p_f64]
     lib.geo_stream_probe.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, _p_f64]
     lib.geo_get_storage_scales.argtypes = [vp, _p_f64, vp]

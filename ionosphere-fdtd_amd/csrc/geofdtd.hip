@@ -123,6 +123,7 @@ struct PlanBase {
   virtual int set_receivers(const int64_t* v, const int64_t* l, const double* w, int32_t n) = 0;
   virtual int sample(void* row, hipStream_t stream) = 0;
   virtual int diagnostics(GeoDiagOut* out, hipStream_t stream) = 0;
+  virtual int radial_profiles(const GeoProfiles* out, hipStream_t stream) = 0;
   virtual int time_kernels(int64_t n, hipStream_t stream, double* out_ms) = 0;
   virtual int get_scales(double* out, hipStream_t stream) = 0;
   virtual int set_scales(const double* in, hipStream_t stream) = 0;
@@ -135,6 +136,8 @@ struct DiagAcc {
   double max_abs;
   double energy;
   long long nonfinite;
+  double loss;
+  double sumsq;
 };
 
 template <typename T, typename C>
@@ -143,11 +146,14 @@ __global__ void diag_field_kernel(const T* __restrict__ x, int64_t rows, int32_t
                                   int32_t field, const double* __restrict__ row_w,
                                   const double* __restrict__ col_w,
                                   const double* __restrict__ eps, int64_t eps_stride,
+                                  const double* __restrict__ sigma, int64_t sigma_stride,
                                   double coef, DiagAcc* __restrict__ partial) {
   __shared__ double s_max[geo::kBlock];
   __shared__ double s_en[geo::kBlock];
   __shared__ long long s_nf[geo::kBlock];
-  double mx = 0.0, en = 0.0;
+  __shared__ double s_ls[geo::kBlock];
+  __shared__ double s_sq[geo::kBlock];
+  double mx = 0.0, en = 0.0, ls = 0.0, sq = 0.0;
   long long nf = 0;
   const C inv_scale = geo::is_narrow<T>() ? (C)dyn->inv_scale[field] : (C)1;
   const int64_t total = rows * cols;
@@ -162,21 +168,68 @@ __global__ void diag_field_kernel(const T* __restrict__ x, int64_t rows, int32_t
     }
     mx = fmax(mx, fabs(v));
     const double e = eps != nullptr ? eps[r * eps_stride + c] : 1.0;
-    en += coef * e * v * v * row_w[r] * col_w[c];
+    const double w = v * v * row_w[r] * col_w[c];
+    en += coef * e * w;
+    if (sigma != nullptr) ls += sigma[r * sigma_stride + c] * w;
+    sq += v * v;
   }
   s_max[threadIdx.x] = mx;
   s_en[threadIdx.x] = en;
   s_nf[threadIdx.x] = nf;
+  s_ls[threadIdx.x] = ls;
+  s_sq[threadIdx.x] = sq;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
       s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + s]);
       s_en[threadIdx.x] += s_en[threadIdx.x + s];
       s_nf[threadIdx.x] += s_nf[threadIdx.x + s];
+      s_ls[threadIdx.x] += s_ls[threadIdx.x + s];
+      s_sq[threadIdx.x] += s_sq[threadIdx.x + s];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) partial[blockIdx.x] = {s_max[0], s_en[0], s_nf[0]};
+  if (threadIdx.x == 0) partial[blockIdx.x] = {s_max[0], s_en[0], s_nf[0], s_ls[0], s_sq[0]};
+}
+
+// Per-layer horizontal sums of one field for the radial profiles: column c of
+// partial block b holds sum_r w_r eps(r, c) x(r, c)^2 (eps = 1 when absent) and
+// sum_r w_r sigma(r, c) x(r, c)^2 over the block's rows; non-finite values are
+// skipped as in diag_field_kernel.  One wave per block, lanes across layers.
+template <typename T, typename C>
+__global__ void diag_profile_kernel(const T* __restrict__ x, int64_t rows, int32_t cols,
+                                    int32_t pitch, const geo::Dynamic* __restrict__ dyn,
+                                    int32_t field, const double* __restrict__ row_w,
+                                    const double* __restrict__ eps, int64_t eps_stride,
+                                    const double* __restrict__ sigma, int64_t sigma_stride,
+                                    int64_t rows_per_block, double* __restrict__ partial) {
+  const C inv_scale = geo::is_narrow<T>() ? (C)dyn->inv_scale[field] : (C)1;
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+  const int64_t r1 = r0 + rows_per_block < rows ? r0 + rows_per_block : rows;
+  for (int32_t c = threadIdx.x; c < cols; c += blockDim.x) {
+    double en = 0.0, ls = 0.0;
+    for (int64_t r = r0; r < r1; ++r) {
+      const double v = (double)geo::load_one<T, C>(x + r * pitch + c, inv_scale);
+      if (!isfinite(v)) continue;
+      const double q = v * v * row_w[r];
+      en += (eps != nullptr ? eps[r * eps_stride + c] : 1.0) * q;
+      if (sigma != nullptr) ls += sigma[r * sigma_stride + c] * q;
+    }
+    partial[((int64_t)blockIdx.x * 2) * cols + c] = en;
+    partial[((int64_t)blockIdx.x * 2 + 1) * cols + c] = ls;
+  }
+}
+
+// Sums the per-block profile partials in block order: out = (2, cols).
+__global__ void diag_profile_reduce_kernel(const double* __restrict__ partial, int32_t nb,
+                                           int32_t cols, double* __restrict__ out) {
+  for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < 2 * cols;
+       i += gridDim.x * blockDim.x) {
+    const int32_t q = i / cols, c = i - q * cols;
+    double s = 0.0;
+    for (int32_t b = 0; b < nb; ++b) s += partial[((int64_t)b * 2 + q) * cols + c];
+    out[i] = s;
+  }
 }
 
 // Discrete div-B per prism cell (f, k) in FP64; partial maxima of |D| and of
@@ -232,26 +285,34 @@ __global__ void diag_reduce_kernel(const DiagAcc* __restrict__ partial, int32_t 
   __shared__ double s_max[geo::kBlock];
   __shared__ double s_en[geo::kBlock];
   __shared__ long long s_nf[geo::kBlock];
-  double mx = 0.0, en = 0.0;
+  __shared__ double s_ls[geo::kBlock];
+  __shared__ double s_sq[geo::kBlock];
+  double mx = 0.0, en = 0.0, ls = 0.0, sq = 0.0;
   long long nf = 0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     mx = fmax(mx, partial[i].max_abs);
     en += partial[i].energy;
     nf += partial[i].nonfinite;
+    ls += partial[i].loss;
+    sq += partial[i].sumsq;
   }
   s_max[threadIdx.x] = mx;
   s_en[threadIdx.x] = en;
   s_nf[threadIdx.x] = nf;
+  s_ls[threadIdx.x] = ls;
+  s_sq[threadIdx.x] = sq;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
     if ((int)threadIdx.x < s) {
       s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + s]);
       s_en[threadIdx.x] += s_en[threadIdx.x + s];
       s_nf[threadIdx.x] += s_nf[threadIdx.x + s];
+      s_ls[threadIdx.x] += s_ls[threadIdx.x + s];
+      s_sq[threadIdx.x] += s_sq[threadIdx.x + s];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = {s_max[0], s_en[0], s_nf[0]};
+  if (threadIdx.x == 0) *out = {s_max[0], s_en[0], s_nf[0], s_ls[0], s_sq[0]};
 }
 
 // Final reduction for the div-B partials: both fields are maxima.
@@ -274,7 +335,7 @@ __global__ void diag_reduce_max_kernel(const DiagAcc* __restrict__ partial, int3
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = {s_a[0], s_b[0], 0};
+  if (threadIdx.x == 0) *out = {s_a[0], s_b[0], 0, 0.0, 0.0};
 }
 
 // ---------------------------------------------------------------------------
@@ -445,6 +506,16 @@ struct Plan final : PlanBase {
   double* eps_er_dev = nullptr;
   double* eps_et_dev = nullptr;
   int64_t eps_er_stride = 0, eps_et_stride = 0;
+  // conductivity (S/m) for the conductive-loss diagnostic; null: no loss
+  double* sig_er_dev = nullptr;
+  double* sig_et_dev = nullptr;
+  int64_t sig_er_stride = 0, sig_et_stride = 0;
+  // radial profiles: per-block partials (kProfileBlocks, 2, R+1), per-field
+  // sums (4, 2, R+1) and the host copies of the radial weights
+  double* prof_partial = nullptr;
+  double* prof_out = nullptr;
+  std::vector<double> col_w_host[4];
+  static constexpr int kProfileBlocks = 512;
   // graph cache
   hipStream_t capture_stream = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -926,6 +997,16 @@ struct Plan final : PlanBase {
       if ((rc = arena.upload(eet, &eps_et_dev)) != GEO_OK) return rc;
       eps_er_stride = (ph->eps_er && er_rows > 1) ? R + 1 : 0;
       eps_et_stride = (ph->eps_et && et_rows > 1) ? R : 0;
+      if (ph->sigma_er) {
+        std::vector<double> s(ph->sigma_er, ph->sigma_er + er_rows * (R + 1));
+        if ((rc = arena.upload(s, &sig_er_dev)) != GEO_OK) return rc;
+        sig_er_stride = er_rows > 1 ? R + 1 : 0;
+      }
+      if (ph->sigma_et) {
+        std::vector<double> s(ph->sigma_et, ph->sigma_et + et_rows * R);
+        if ((rc = arena.upload(s, &sig_et_dev)) != GEO_OK) return rc;
+        sig_et_stride = et_rows > 1 ? R : 0;
+      }
       std::vector<double> er_row(V), er_col(R + 1), et_row(E), et_col(R), hr_row(F), hr_col(R),
           ht_row(E), ht_col(R + 1);
       for (int64_t v = 0; v < V; ++v) er_row[v] = m->dual_cell_solid_angles[v];
@@ -942,6 +1023,16 @@ struct Plan final : PlanBase {
       if ((rc = arena.upload(hr_col, &w_hr_col)) != GEO_OK) return rc;
       if ((rc = arena.upload(ht_row, &w_ht_row)) != GEO_OK) return rc;
       if ((rc = arena.upload(ht_col, &w_ht_col)) != GEO_OK) return rc;
+      col_w_host[0] = er_col;
+      col_w_host[1] = et_col;
+      col_w_host[2] = hr_col;
+      col_w_host[3] = ht_col;
+      void* pp = nullptr;
+      if ((rc = arena.zeroed(sizeof(double) * (size_t)(kProfileBlocks + 4) * 2 * (R + 1), &pp)) !=
+          GEO_OK)
+        return rc;
+      prof_partial = static_cast<double*>(pp);
+      prof_out = prof_partial + (size_t)kProfileBlocks * 2 * (R + 1);
       void* p = nullptr;
       if ((rc = arena.zeroed(sizeof(DiagAcc) * (kDiagBlocks * 5 + 5), &p)) != GEO_OK) return rc;
       diag_partial = static_cast<DiagAcc*>(p);
@@ -1134,12 +1225,12 @@ struct Plan final : PlanBase {
 
   int diag_one(const T* x, int64_t rows, int32_t cols, int32_t pitch, int32_t field,
                const double* row_w, const double* col_w, const double* eps, int64_t eps_stride,
-               double coef, int slot, hipStream_t stream) {
+               const double* sigma, int64_t sigma_stride, double coef, int slot,
+               hipStream_t stream) {
     DiagAcc* part = diag_partial + slot * kDiagBlocks;
     hipLaunchKernelGGL((diag_field_kernel<T, C>), dim3(kDiagBlocks), dim3(geo::kBlock), 0,
                        stream, x, rows, cols, pitch, (const geo::Dynamic*)dyn, field, row_w,
-                       col_w, eps, eps_stride,
-                       coef, part);
+                       col_w, eps, eps_stride, sigma, sigma_stride, coef, part);
     hipLaunchKernelGGL(diag_reduce_kernel, dim3(1), dim3(geo::kBlock), 0, stream, part,
                        kDiagBlocks, diag_out + slot);
     GEO_HIP(hipGetLastError());
@@ -1150,16 +1241,18 @@ struct Plan final : PlanBase {
     if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
     int rc;
     if ((rc = diag_one(hp.er, V, (int32_t)(R + 1), (int32_t)pn, geo::kFieldEr, w_er_row,
-                       w_er_col, eps_er_dev, eps_er_stride, 0.5, 0, stream)) != GEO_OK)
+                       w_er_col, eps_er_dev, eps_er_stride, sig_er_dev, sig_er_stride, 0.5, 0,
+                       stream)) != GEO_OK)
       return rc;
     if ((rc = diag_one(hp.et, E, (int32_t)R, (int32_t)pm, geo::kFieldEt, w_et_row, w_et_col,
-                       eps_et_dev, eps_et_stride, 0.5, 1, stream)) != GEO_OK)
+                       eps_et_dev, eps_et_stride, sig_et_dev, sig_et_stride, 0.5, 1,
+                       stream)) != GEO_OK)
       return rc;
     if ((rc = diag_one(hp.hr, F, (int32_t)R, (int32_t)pm, geo::kFieldHr, w_hr_row, w_hr_col,
-                       nullptr, 0, 0.5 * kMu0, 2, stream)) != GEO_OK)
+                       nullptr, 0, nullptr, 0, 0.5 * kMu0, 2, stream)) != GEO_OK)
       return rc;
     if ((rc = diag_one(hp.ht, E, (int32_t)(R + 1), (int32_t)pn, geo::kFieldHt, w_ht_row,
-                       w_ht_col, nullptr, 0, 0.5 * kMu0, 3, stream)) != GEO_OK)
+                       w_ht_col, nullptr, 0, nullptr, 0, 0.5 * kMu0, 3, stream)) != GEO_OK)
       return rc;
     {
       DiagAcc* part = diag_partial + 4 * kDiagBlocks;
@@ -1190,6 +1283,64 @@ struct Plan final : PlanBase {
                      host[3].nonfinite;
     out->divb_max_abs = host[4].max_abs;
     out->divb_flux_scale = host[4].energy;
+    out->loss_er = host[0].loss;
+    out->loss_et = host[1].loss;
+    out->sumsq_er = host[0].sumsq;
+    out->sumsq_et = host[1].sumsq;
+    out->sumsq_hr = host[2].sumsq;
+    out->sumsq_ht = host[3].sumsq;
+    out->nonfinite_er = host[0].nonfinite;
+    out->nonfinite_et = host[1].nonfinite;
+    out->nonfinite_hr = host[2].nonfinite;
+    out->nonfinite_ht = host[3].nonfinite;
+    return GEO_OK;
+  }
+
+  int radial_profiles(const GeoProfiles* out, hipStream_t stream) override {
+    if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
+    struct Field {
+      const T* x;
+      int64_t rows;
+      int32_t cols, pitch, id;
+      const double *row_w, *eps, *sigma;
+      int64_t eps_stride, sigma_stride;
+      double coef;
+      double *energy, *loss;
+    };
+    const Field fields[4] = {
+        {hp.er, V, (int32_t)(R + 1), (int32_t)pn, geo::kFieldEr, w_er_row, eps_er_dev,
+         sig_er_dev, eps_er_stride, sig_er_stride, 0.5, out->energy_er, out->loss_er},
+        {hp.et, E, (int32_t)R, (int32_t)pm, geo::kFieldEt, w_et_row, eps_et_dev, sig_et_dev,
+         eps_et_stride, sig_et_stride, 0.5, out->energy_et, out->loss_et},
+        {hp.hr, F, (int32_t)R, (int32_t)pm, geo::kFieldHr, w_hr_row, nullptr, nullptr, 0, 0,
+         0.5 * kMu0, out->energy_hr, nullptr},
+        {hp.ht, E, (int32_t)(R + 1), (int32_t)pn, geo::kFieldHt, w_ht_row, nullptr, nullptr, 0,
+         0, 0.5 * kMu0, out->energy_ht, nullptr}};
+    const int64_t stride = 2 * (R + 1);
+    for (int f = 0; f < 4; ++f) {
+      const Field& d = fields[f];
+      if (!d.energy && !d.loss) continue;
+      const int32_t nb = (int32_t)std::min<int64_t>(kProfileBlocks, d.rows);
+      const int64_t per = (d.rows + nb - 1) / nb;
+      hipLaunchKernelGGL((diag_profile_kernel<T, C>), dim3(nb), dim3(64), 0, stream, d.x,
+                         d.rows, d.cols, d.pitch, (const geo::Dynamic*)dyn, d.id, d.row_w,
+                         d.eps, d.eps_stride, d.sigma, d.sigma_stride, per, prof_partial);
+      hipLaunchKernelGGL(diag_profile_reduce_kernel, dim3(1), dim3(geo::kBlock), 0, stream,
+                         prof_partial, nb, d.cols, prof_out + f * stride);
+      GEO_HIP(hipGetLastError());
+    }
+    std::vector<double> host(4 * stride);
+    GEO_HIP(hipMemcpyAsync(host.data(), prof_out, sizeof(double) * host.size(),
+                           hipMemcpyDeviceToHost, stream));
+    GEO_HIP(hipStreamSynchronize(stream));
+    for (int f = 0; f < 4; ++f) {
+      const Field& d = fields[f];
+      const double* sums = host.data() + f * stride;
+      for (int32_t c = 0; c < d.cols; ++c) {
+        if (d.energy) d.energy[c] = d.coef * sums[c] * col_w_host[f][c];
+        if (d.loss) d.loss[c] = sums[d.cols + c] * col_w_host[f][c];
+      }
+    }
     return GEO_OK;
   }
 
@@ -1733,6 +1884,11 @@ int geo_sample_er(GeoPlan* plan, void* row, void* stream) {
 int geo_diagnostics(GeoPlan* plan, GeoDiagOut* out, void* stream) {
   if (!plan || !out) return fail(GEO_ERR_INVALID, "null argument");
   return plan->impl->diagnostics(out, static_cast<hipStream_t>(stream));
+}
+
+int geo_radial_profiles(GeoPlan* plan, const GeoProfiles* out, void* stream) {
+  if (!plan || !out) return fail(GEO_ERR_INVALID, "null argument");
+  return plan->impl->radial_profiles(out, static_cast<hipStream_t>(stream));
 }
 
 int geo_time_kernels(GeoPlan* plan, int64_t n_steps, void* stream, double* out_ms) {

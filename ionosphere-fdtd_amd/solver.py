@@ -376,10 +376,12 @@ class GeodesicFDTD:
         rd = _native.GeoRadialDesc(radial_cells=R, radii_m=f64(self.radii_m),
                                    full_spherical=int(self.config.geometry_mode ==
                                                       "full-spherical"))
-        ca_er, cb_er, eps_er = self._rows(self._ca_er, self._cb_er,
-                                          np.broadcast_to(self._eps_er, self.sigma_er.shape))
-        ca_et, cb_et, eps_et = self._rows(self._ca_et, self._cb_et,
-                                          np.broadcast_to(self._eps_et, self.sigma_et.shape))
+        ca_er, cb_er, eps_er, sig_er = self._rows(
+            self._ca_er, self._cb_er, np.broadcast_to(self._eps_er, self.sigma_er.shape),
+            self.sigma_er)
+        ca_et, cb_et, eps_et, sig_et = self._rows(
+            self._ca_et, self._cb_et, np.broadcast_to(self._eps_et, self.sigma_et.shape),
+            self.sigma_et)
         storage = self.backend.dtype_name
         narrow = storage in ("float16", "bfloat16")
         ph = _native.GeoPhysicsDesc(
@@ -387,6 +389,7 @@ class GeodesicFDTD:
             ca_er=f64(ca_er), cb_er=f64(cb_er), er_coef_rows=ca_er.shape[0],
             ca_et=f64(ca_et), cb_et=f64(cb_et), et_coef_rows=ca_et.shape[0],
             eps_er=f64(eps_er), eps_et=f64(eps_et),
+            sigma_er=f64(sig_er), sigma_et=f64(sig_et),
             scaling=int(narrow and self.storage_scaling == "dynamic"),
             scale_er=self._initial_scales["er"] if narrow else 1.0,
             scale_et=self._initial_scales["et"] if narrow else 1.0,
@@ -774,6 +777,19 @@ class GeodesicFDTD:
         _native.check(self._lib.geo_diagnostics(self._plan, C.byref(out), self._stream()),
                       self._lib)
         return {name: getattr(out, name) for name, _ in out._fields_}
+
+    def radial_profiles(self) -> dict[str, np.ndarray]:
+        """Per-layer FP64 energy (J) and conductive loss (W) of each field:
+        the horizontal sums of verification/physics_diagnostics/model.py:242-267
+        times the radial weights (r^2 L_k at nodes, rbar^2 dr at midpoints)."""
+        R = len(self.radial_midpoints_m)
+        out = {"energy_er": np.zeros(R + 1), "energy_et": np.zeros(R),
+               "energy_hr": np.zeros(R), "energy_ht": np.zeros(R + 1),
+               "loss_er": np.zeros(R + 1), "loss_et": np.zeros(R)}
+        desc = _native.GeoProfiles(**{k: _native.f64_ptr(v) for k, v in out.items()})
+        _native.check(self._lib.geo_radial_profiles(self._plan, C.byref(desc), self._stream()),
+                      self._lib)
+        return out
 
     def energy_j(self) -> float:
         """Total staggered field energy (physics_diagnostics/model.py:242-258 weights)."""
