@@ -276,6 +276,26 @@ int geo_ensemble_tune(GeoEnsemble* ens, int32_t steps, void* stream, int32_t* st
 int geo_ensemble_step(GeoEnsemble* ens, const void* currents, int64_t n_steps, int32_t chunk,
                       void* traces, void* stream);
 void geo_ensemble_destroy(GeoEnsemble* ens);
+/* Surface operators of the backend interface (backends/base.py:38-56, with
+ * the semantics and summation order of numpy_backend.py:90-114) on device
+ * arrays with `cols` contiguous values per entity, FP32 (value_bytes 4) or
+ * FP64 (8).  The operator context keeps the mesh topology on the device.
+ *   GEO_OP_EDGE_DIFFERENCE       (V,cols) -> (E,cols)  v[head] - v[tail]
+ *   GEO_OP_DUAL_EDGE_DIFFERENCE  (F,cols) -> (E,cols)  f[left] - f[right]
+ *   GEO_OP_FACE_CIRCULATION      (E,cols) -> (F,cols)  sum_c s_fc e[face_edges]
+ *   GEO_OP_DUAL_CELL_CIRCULATION (E,cols) -> (V,cols)  sum_slot s e[vertex_edges]
+ * geo_array_diff is backend.diff of a (rows, cols) array along axis 0 or 1. */
+#define GEO_OP_EDGE_DIFFERENCE 0
+#define GEO_OP_DUAL_EDGE_DIFFERENCE 1
+#define GEO_OP_FACE_CIRCULATION 2
+#define GEO_OP_DUAL_CELL_CIRCULATION 3
+typedef struct GeoMeshOps GeoMeshOps;
+int geo_mesh_ops_create(const GeoMeshDesc* mesh, GeoMeshOps** out);
+int geo_mesh_op(GeoMeshOps* ops, int32_t op, int32_t value_bytes, const void* in, int64_t cols,
+                void* out, void* stream);
+int geo_array_diff(int32_t value_bytes, const void* in, int64_t rows, int64_t cols, int32_t axis,
+                   void* out, void* stream);
+void geo_mesh_ops_destroy(GeoMeshOps* ops);
 const char* geo_last_error(void);
 int geo_version(void);
 

@@ -113,6 +113,7 @@ EXPORTED_SYMBOLS = (
     "geo_last_error", "geo_version", "geo_stream_probe", "geo_get_storage_scales",
     "geo_set_storage_scales", "geo_ensemble_create", "geo_ensemble_configure",
     "geo_ensemble_tune", "geo_ensemble_step", "geo_ensemble_destroy",
+    "geo_mesh_ops_create", "geo_mesh_op", "geo_array_diff", "geo_mesh_ops_destroy",
     # include/geosurrogate.h
     "geo_gemm_bf16", "geo_f32_to_bf16", "geo_silu_forward", "geo_silu_backward",
     "geo_column_sums", "geo_im2col_up", "geo_col2im_up", "geo_adam_step", "geo_trace_loss",
@@ -154,6 +155,11 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.geo_ensemble_step.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp]
     lib.geo_ensemble_destroy.argtypes = [vp]
     lib.geo_ensemble_destroy.restype = None
+    lib.geo_mesh_ops_create.argtypes = [C.POINTER(GeoMeshDesc), C.POINTER(vp)]
+    lib.geo_mesh_op.argtypes = [vp, C.c_int32, C.c_int32, vp, C.c_int64, vp, vp]
+    lib.geo_array_diff.argtypes = [C.c_int32, vp, C.c_int64, C.c_int64, C.c_int32, vp, vp]
+    lib.geo_mesh_ops_destroy.argtypes = [vp]
+    lib.geo_mesh_ops_destroy.restype = None
     i64, i32, f32 = C.c_int64, C.c_int32, C.c_float
     lib.geo_gemm_bf16.argtypes = [i64, i64, i64, vp, i64, i64, vp, i64, i64, vp, vp, vp, i64,
                                   i32, vp]

@@ -1797,6 +1797,62 @@ int check_device() {
 
 int geo_internal_fail(int code, const char* msg) { return fail(code, msg); }
 
+// ---------------------------------------------------------------------------
+// Surface operators of the backend interface (numpy_backend.py:90-114): one
+// thread per output value, the reference's order of additions.
+// ---------------------------------------------------------------------------
+template <typename V>
+__global__ void mesh_op_kernel(int32_t op, const V* __restrict__ in, int64_t out_rows,
+                               int64_t cols, const int32_t* __restrict__ idx,
+                               const int32_t* __restrict__ sgn, int32_t width,
+                               V* __restrict__ out) {
+  const int64_t total = out_rows * cols;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / cols;
+    const int64_t c = i - r * cols;
+    V acc;
+    if (op == GEO_OP_EDGE_DIFFERENCE || op == GEO_OP_DUAL_EDGE_DIFFERENCE) {
+      // edges (E,2) = tail, head; edge faces (E,2) = left, right
+      const V a = in[(int64_t)idx[2 * r] * cols + c], b = in[(int64_t)idx[2 * r + 1] * cols + c];
+      acc = op == GEO_OP_EDGE_DIFFERENCE ? b - a : a - b;
+    } else {
+      // face_edges (F,3) with signs, or padded vertex incidence (V,width)
+      acc = in[(int64_t)idx[width * r] * cols + c] * (V)sgn[width * r];
+      for (int32_t s = 1; s < width; ++s)
+        acc += in[(int64_t)idx[width * r + s] * cols + c] * (V)sgn[width * r + s];
+    }
+    out[i] = acc;
+  }
+}
+
+template <typename V>
+__global__ void array_diff_kernel(const V* __restrict__ in, int64_t rows, int64_t cols,
+                                  int32_t axis, V* __restrict__ out) {
+  const int64_t orows = axis == 0 ? rows - 1 : rows;
+  const int64_t ocols = axis == 0 ? cols : cols - 1;
+  const int64_t total = orows * ocols;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = i / ocols;
+    const int64_t c = i - r * ocols;
+    const int64_t a = r * cols + c;
+    out[i] = in[axis == 0 ? a + cols : a + 1] - in[a];
+  }
+}
+
+struct GeoMeshOpsImpl {
+  int64_t V = 0, E = 0, F = 0;
+  int32_t width = 0;
+  DeviceArena arena;
+  int32_t *edges = nullptr, *edge_faces = nullptr, *face_edges = nullptr, *face_signs = nullptr;
+  int32_t *vertex_edges = nullptr, *vertex_signs = nullptr;
+};
+
+struct GeoMeshOps {
+  GeoMeshOpsImpl impl;
+};
+
 struct GeoPlan {
   std::unique_ptr<PlanBase> impl;
 };
@@ -1808,6 +1864,111 @@ struct GeoEnsemble {
 extern "C" {
 
 int geo_version(void) { return 1; }
+
+int geo_mesh_ops_create(const GeoMeshDesc* m, GeoMeshOps** out) {
+  if (!m || !out) return fail(GEO_ERR_INVALID, "null argument");
+  *out = nullptr;
+  int rc = check_device();
+  if (rc != GEO_OK) return rc;
+  if (!m->edges || !m->face_edges || !m->face_edge_signs || !m->edge_left_faces ||
+      !m->edge_right_faces || !m->vertex_edges || !m->vertex_edge_signs)
+    return fail(GEO_ERR_INVALID, "mesh topology arrays are required");
+  const int64_t V = m->n_vertices, E = m->n_edges, F = m->n_faces, W = m->max_degree;
+  if (V <= 0 || E <= 0 || F <= 0 || W < 1 || W > 16 || V > INT32_MAX || E > INT32_MAX ||
+      F > INT32_MAX)
+    return fail(GEO_ERR_INVALID, "mesh sizes are out of range");
+  std::vector<int32_t> edges(2 * E), faces2(2 * E), fe(3 * F), fs(3 * F), ve(W * V), vs(W * V);
+  for (int64_t e = 0; e < E; ++e) {
+    const int64_t t = m->edges[2 * e], h = m->edges[2 * e + 1];
+    const int64_t l = m->edge_left_faces[e], r = m->edge_right_faces[e];
+    if (t < 0 || t >= V || h < 0 || h >= V || l < 0 || l >= F || r < 0 || r >= F)
+      return fail(GEO_ERR_INVALID, "edge incidence index out of range");
+    edges[2 * e] = (int32_t)t;
+    edges[2 * e + 1] = (int32_t)h;
+    faces2[2 * e] = (int32_t)l;
+    faces2[2 * e + 1] = (int32_t)r;
+  }
+  for (int64_t i = 0; i < 3 * F; ++i) {
+    if (m->face_edges[i] < 0 || m->face_edges[i] >= E)
+      return fail(GEO_ERR_INVALID, "face edge index out of range");
+    fe[i] = (int32_t)m->face_edges[i];
+    fs[i] = (int32_t)m->face_edge_signs[i];
+  }
+  for (int64_t i = 0; i < W * V; ++i) {
+    if (m->vertex_edges[i] < 0 || m->vertex_edges[i] >= E)
+      return fail(GEO_ERR_INVALID, "vertex edge index out of range");
+    ve[i] = (int32_t)m->vertex_edges[i];
+    vs[i] = (int32_t)m->vertex_edge_signs[i];
+  }
+  auto ops = std::make_unique<GeoMeshOps>();
+  GeoMeshOpsImpl& o = ops->impl;
+  o.V = V;
+  o.E = E;
+  o.F = F;
+  o.width = (int32_t)W;
+  if ((rc = o.arena.upload(edges, &o.edges)) != GEO_OK) return rc;
+  if ((rc = o.arena.upload(faces2, &o.edge_faces)) != GEO_OK) return rc;
+  if ((rc = o.arena.upload(fe, &o.face_edges)) != GEO_OK) return rc;
+  if ((rc = o.arena.upload(fs, &o.face_signs)) != GEO_OK) return rc;
+  if ((rc = o.arena.upload(ve, &o.vertex_edges)) != GEO_OK) return rc;
+  if ((rc = o.arena.upload(vs, &o.vertex_signs)) != GEO_OK) return rc;
+  *out = ops.release();
+  return GEO_OK;
+}
+
+int geo_mesh_op(GeoMeshOps* ops, int32_t op, int32_t value_bytes, const void* in, int64_t cols,
+                void* out, void* stream) {
+  if (!ops || !in || !out) return fail(GEO_ERR_INVALID, "null argument");
+  if (value_bytes != 4 && value_bytes != 8) return fail(GEO_ERR_INVALID, "values must be FP32 or FP64");
+  if (cols < 1) return fail(GEO_ERR_INVALID, "cols must be positive");
+  const GeoMeshOpsImpl& o = ops->impl;
+  int64_t rows;
+  const int32_t *idx, *sgn = nullptr;
+  int32_t width = 0;
+  switch (op) {
+    case GEO_OP_EDGE_DIFFERENCE: rows = o.E; idx = o.edges; break;
+    case GEO_OP_DUAL_EDGE_DIFFERENCE: rows = o.E; idx = o.edge_faces; break;
+    case GEO_OP_FACE_CIRCULATION: rows = o.F; idx = o.face_edges; sgn = o.face_signs; width = 3; break;
+    case GEO_OP_DUAL_CELL_CIRCULATION:
+      rows = o.V; idx = o.vertex_edges; sgn = o.vertex_signs; width = o.width; break;
+    default: return fail(GEO_ERR_INVALID, "unknown mesh operator");
+  }
+  const int64_t total = rows * cols;
+  const int blocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (value_bytes == 8)
+    hipLaunchKernelGGL(mesh_op_kernel<double>, dim3(blocks), dim3(256), 0, s, op,
+                       static_cast<const double*>(in), rows, cols, idx, sgn, width,
+                       static_cast<double*>(out));
+  else
+    hipLaunchKernelGGL(mesh_op_kernel<float>, dim3(blocks), dim3(256), 0, s, op,
+                       static_cast<const float*>(in), rows, cols, idx, sgn, width,
+                       static_cast<float*>(out));
+  GEO_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+int geo_array_diff(int32_t value_bytes, const void* in, int64_t rows, int64_t cols, int32_t axis,
+                   void* out, void* stream) {
+  if (!in || !out) return fail(GEO_ERR_INVALID, "null argument");
+  if (value_bytes != 4 && value_bytes != 8) return fail(GEO_ERR_INVALID, "values must be FP32 or FP64");
+  if (axis != 0 && axis != 1) return fail(GEO_ERR_INVALID, "axis must be 0 or 1");
+  if (rows < 1 || cols < 1 || (axis == 0 ? rows : cols) < 2)
+    return fail(GEO_ERR_INVALID, "array is too small to difference");
+  const int64_t total = (axis == 0 ? rows - 1 : rows) * (axis == 0 ? cols : cols - 1);
+  const int blocks = (int)std::min<int64_t>((total + 255) / 256, 8192);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (value_bytes == 8)
+    hipLaunchKernelGGL(array_diff_kernel<double>, dim3(blocks), dim3(256), 0, s,
+                       static_cast<const double*>(in), rows, cols, axis, static_cast<double*>(out));
+  else
+    hipLaunchKernelGGL(array_diff_kernel<float>, dim3(blocks), dim3(256), 0, s,
+                       static_cast<const float*>(in), rows, cols, axis, static_cast<float*>(out));
+  GEO_HIP(hipGetLastError());
+  return GEO_OK;
+}
+
+void geo_mesh_ops_destroy(GeoMeshOps* ops) { delete ops; }
 
 const char* geo_last_error(void) { return g_error.c_str(); }
 
