@@ -113,3 +113,26 @@ def test_profiles_match_fp64_evaluation_of_the_reference_formulas(dtype, rtol):
     d = sim.field_diagnostics()
     assert d["loss_er"] + d["loss_et"] == pytest.approx(
         snap.scalars["conductive_loss/total_w"], rel=1e-10)
+
+
+def test_injected_nan_clears_the_finite_flag_of_that_field_only():
+    """A NaN written into one Er value (the reference's failure-detection
+    check): that field reports field_finite 0 and a NaN RMS, the device count
+    sees one non-finite value, and every other field stays finite."""
+    import ionosphere_fdtd_amd as g
+
+    sc = S.scenario("c0")
+    sim = S.build_sim(sc)
+    rng = np.random.default_rng(3)
+    for name in ("er", "et", "hr", "ht"):
+        setattr(sim, name, rng.standard_normal(tuple(getattr(sim, name).shape)))
+    sim.er[5, 3] = float("nan")
+    snap = g.PhysicsDiagnosticSampler(sim).sample()
+    assert snap.scalars["field_finite/er"] == 0.0
+    assert math.isnan(snap.scalars["field_rms/er"])
+    for name in ("et", "hr", "ht"):
+        assert snap.scalars[f"field_finite/{name}"] == 1.0
+        assert math.isfinite(snap.scalars[f"field_rms/{name}"])
+    d = sim.field_diagnostics()
+    assert d["nonfinite"] == 1 and d["nonfinite_er"] == 1
+    assert not sim.diagnostics()["field_finite"]
