@@ -847,6 +847,31 @@ class OracleSim:
             self.steps += 1
             self.time_s = self.steps * self.time_step_s
 
+    def record_h_observations(self, faces, face_layers, face_weights, edges, edge_layers,
+                              edge_weights, steps, sample_every=1):
+        """solver.py:1138-1229: the first row is H at -1/2 and the row of sample
+        step k is H at k - 1/2; samples at 0, sample_every, ... and at steps."""
+        faces = np.asarray(faces, dtype=np.int64)
+        face_layers = np.asarray(face_layers, dtype=np.int64)
+        fw = np.asarray(face_weights, dtype=np.float64).astype(self.dtype)
+        edges = np.asarray(edges, dtype=np.int64)
+        edge_layers = np.asarray(edge_layers, dtype=np.int64)
+        ew = np.asarray(edge_weights, dtype=np.float64).astype(self.dtype)
+        sample_steps = np.unique(np.concatenate((np.arange(0, steps + 1, sample_every),
+                                                 np.asarray((steps,)))))
+        rt = np.zeros((len(sample_steps), faces.shape[0]), dtype=self.dtype)
+        tt = np.zeros((len(sample_steps), edges.shape[0]), dtype=self.dtype)
+
+        def sample(row):
+            rt[row] = (self.hr[faces, face_layers] * fw).sum(axis=1)
+            tt[row] = (self.ht[edges, edge_layers] * ew).sum(axis=1)
+
+        sample(0)
+        for row, target in enumerate(sample_steps[1:], start=1):
+            self.step(int(target) - self.steps)
+            sample(row)
+        return rt, tt
+
     def record_er_observations(self, vertices, layers, weights, steps):
         """solver.py:1080-1136: row 0 initial, row n after step n."""
         vertices = np.asarray(vertices, dtype=np.int64)

@@ -322,3 +322,30 @@ def test_div_b_detects_a_non_solenoidal_state():
     sim.hr = rng.standard_normal(tuple(sim.hr.shape))
     d = sim.field_diagnostics()
     assert d["divb_max_abs"] > 1e-3 * d["divb_flux_scale"]
+
+
+def test_h_observations_match_oracle_fp64():
+    """record_h_observations (solver.py:1138-1229): weighted Hr / Ht samples at
+    the half-step times, every third step plus the last, against the oracle."""
+    sc = S.scenario("c0_pulse")
+    sim = S.build_sim(sc)
+    ref = S.build_oracle(sc)
+    mesh = sim.mesh
+    rng = np.random.default_rng(21)
+    R = len(sim.radial_midpoints_m)
+    faces = rng.integers(0, mesh.n_faces, (5, 3))
+    flayers = rng.integers(0, R, (5, 3))
+    fw = rng.random((5, 3))
+    edges = rng.integers(0, mesh.n_edges, (4, 2))
+    elayers = rng.integers(0, R + 1, (4, 2))
+    ew = rng.random((4, 2))
+    got_r, got_t = sim.record_h_observations(faces, flayers, fw, edges, elayers, ew, 100,
+                                             sample_every=3)
+    want_r, want_t = ref.record_h_observations(faces, flayers, fw, edges, elayers, ew, 100,
+                                               sample_every=3)
+    assert got_r.shape == want_r.shape == (35, 5) and got_t.shape == want_t.shape == (35, 4)
+    scale = max(float(np.max(np.abs(want_r))), float(np.max(np.abs(want_t))))
+    assert scale > 0
+    np.testing.assert_allclose(got_r, want_r, rtol=0, atol=FP64_BAR * scale)
+    np.testing.assert_allclose(got_t, want_t, rtol=0, atol=FP64_BAR * scale)
+    assert sim.steps == ref.steps == 100
