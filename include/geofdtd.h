@@ -247,10 +247,13 @@ int geo_get_storage_scales(GeoPlan* plan, double* out4, void* stream);
 int geo_set_storage_scales(GeoPlan* plan, const double* in4, void* stream);
 void geo_plan_destroy(GeoPlan* plan);
 /* Bandwidth probe (no counterpart in the reference): average time (ms) of a
- * 16-byte-strip kernel over `bytes` -- mode 0 copy src->dst, 1 read src,
- * 2 write dst, +4 with non-temporal hints.  The measured copy rate is the
- * achievable HBM ceiling reported beside the roofline, and its rocprofv3
- * FETCH_SIZE / WRITE_SIZE readings calibrate the counters. */
+ * kernel moving one 16-byte strip per thread over `bytes` -- mode 0 copy src->dst, 1 read src,
+ * 2 write dst, +4 with non-temporal hints; mode 8 is the access pattern of
+ * the H kernel's edge strips without arithmetic (rows of 68 floats in dst
+ * updated in place from two gathered rows of src, a third as many rows).
+ * The measured rates are the achievable HBM ceilings reported beside the
+ * roofline, and the copy's rocprofv3 FETCH_SIZE / WRITE_SIZE readings
+ * calibrate the counters. */
 int geo_stream_probe(void* dst, const void* src, int64_t bytes, int32_t mode, int32_t repeats,
                      void* stream, double* out_ms);
 /* Ensembles (no reference counterpart: the reference advances one

@@ -1400,7 +1400,7 @@ struct Plan final : PlanBase {
   }
 };
 
-// Bandwidth probe: 16-byte strips, grid-stride.  mode 0 copy, 1 read (sum
+// Bandwidth probe: 16-byte strips, one per thread.  mode 0 copy, 1 read (sum
 // into a per-thread register written only when the impossible sentinel
 // matches, so the loads are not removed), 2 write; +4 = non-temporal hints.
 typedef float f4v __attribute__((ext_vector_type(4)));
@@ -1433,6 +1433,31 @@ __global__ void __launch_bounds__(geo::kBlock) stream_probe_kernel(f4v* __restri
   }
 }
 
+// Access-pattern probe (mode 8): the H kernel's edge strips without their
+// arithmetic.  dst holds rows of 68 floats (17 strips, the s7 x 64 Ht pitch),
+// each updated in place from two gathered rows of src (a third as many rows,
+// like Er for Ht): src row = row / 3 + a hashed offset in [-64, 64], clamped.
+// Threads map to (row, strip) as in the step kernels and the own row carries
+// non-temporal hints.  Compulsory bytes: dst read + written, src read once.
+__global__ void __launch_bounds__(geo::kBlock) pattern_probe_kernel(f4v* __restrict__ dst,
+                                                                    const f4v* __restrict__ src,
+                                                                    int64_t rows) {
+  constexpr int kStrips = 17, kRowsPerBlock = geo::kBlock / kStrips;
+  const int local_row = threadIdx.x / kStrips;
+  const int strip = threadIdx.x - local_row * kStrips;
+  const int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + local_row;
+  if (local_row >= kRowsPerBlock || row >= rows) return;
+  const int64_t vrows = rows / 3 > 0 ? rows / 3 : 1;
+  uint32_t h = (uint32_t)row * 2654435761u;
+  const int64_t a = std::min<int64_t>(vrows - 1, std::max<int64_t>(0, row / 3 + (int64_t)(h >> 25) - 64));
+  h = h * 1597334677u + 12345u;
+  const int64_t b = std::min<int64_t>(vrows - 1, std::max<int64_t>(0, row / 3 + (int64_t)(h >> 25) - 64));
+  const f4v x = src[a * kStrips + strip], y = src[b * kStrips + strip];
+  f4v* own = dst + row * kStrips + strip;
+  f4v v = __builtin_nontemporal_load(own);
+  v += 0.5f * (x - y);
+  __builtin_nontemporal_store(v, own);
+}
 
 // ---------------------------------------------------------------------------
 // Ensembles of independent runs on one mesh (uneven radial grids, own
@@ -2118,12 +2143,34 @@ int geo_stream_probe(void* dst, const void* src, int64_t bytes, int32_t mode, in
   if (!dst || bytes <= 0 || (bytes & 15) || repeats < 1 || !out_ms)
     return fail(GEO_ERR_INVALID, "stream probe needs a 16-byte multiple and a destination");
   if ((mode & 3) != 2 && !src) return fail(GEO_ERR_INVALID, "stream probe needs a source");
+  if (mode != 8 && (mode & ~7)) return fail(GEO_ERR_INVALID, "unknown stream probe mode");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  int dev = 0, cus = 0;
-  GEO_HIP(hipGetDevice(&dev));
-  GEO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  if (mode == 8) {  // edge-strip access pattern over rows of 272 bytes
+    const int64_t rows = bytes / 272;
+    if (rows < 3) return fail(GEO_ERR_INVALID, "pattern probe needs at least 3 rows");
+    const int64_t grid = (rows + geo::kBlock / 17 - 1) / (geo::kBlock / 17);
+    hipEvent_t a, b;
+    GEO_HIP(hipEventCreate(&a));
+    GEO_HIP(hipEventCreate(&b));
+    hipLaunchKernelGGL(pattern_probe_kernel, dim3((unsigned)grid), dim3(geo::kBlock), 0, s,
+                       static_cast<f4v*>(dst), static_cast<const f4v*>(src), rows);
+    GEO_HIP(hipEventRecord(a, s));
+    for (int r = 0; r < repeats; ++r)
+      hipLaunchKernelGGL(pattern_probe_kernel, dim3((unsigned)grid), dim3(geo::kBlock), 0, s,
+                         static_cast<f4v*>(dst), static_cast<const f4v*>(src), rows);
+    GEO_HIP(hipEventRecord(b, s));
+    GEO_HIP(hipEventSynchronize(b));
+    float ms = 0.f;
+    GEO_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *out_ms = ms / repeats;
+    return GEO_OK;
+  }
   const int64_t n = bytes / 16;
-  const int grid = (int)std::min<int64_t>((n + geo::kBlock - 1) / geo::kBlock, (int64_t)cus * 64);
+  // one strip per thread over the whole range, as the step kernels do (a
+  // resident-sized grid-stride loop measured 5.07 against 6.65 TB/s copy)
+  const int grid = (int)std::min<int64_t>((n + geo::kBlock - 1) / geo::kBlock, (int64_t)1 << 30);
   hipEvent_t a, b;
   GEO_HIP(hipEventCreate(&a));
   GEO_HIP(hipEventCreate(&b));
