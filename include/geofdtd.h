@@ -249,7 +249,8 @@ void geo_plan_destroy(GeoPlan* plan);
 /* Bandwidth probe (no counterpart in the reference): average time (ms) of a
  * kernel moving one 16-byte strip per thread over `bytes` -- mode 0 copy src->dst, 1 read src,
  * 2 write dst, +4 with non-temporal hints; mode 8 is the access pattern of
- * the H kernel's edge strips without arithmetic (rows of 68 floats in dst
+ * the H kernel's edge strips without arithmetic, in the step kernels' XCD block
+ * order (rows of 68 floats in dst
  * updated in place from two gathered rows of src, a third as many rows).
  * The measured rates are the achievable HBM ceilings reported beside the
  * roofline, and the copy's rocprofv3 FETCH_SIZE / WRITE_SIZE readings

@@ -188,7 +188,7 @@ def stream_probe(torch, bytes_: int, mode: int, repeats: int = 10) -> float:
     lib = load_library()
     n = bytes_ // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
-    dst = torch.empty_like(src)
+    dst = torch.zeros_like(src)
     out = C.c_double()
     check(lib.geo_stream_probe(dst.data_ptr(), src.data_ptr(), n * 4, mode, repeats,
                                C.c_void_p(torch.cuda.current_stream().cuda_stream),

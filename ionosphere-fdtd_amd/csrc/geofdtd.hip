@@ -1437,15 +1437,16 @@ __global__ void __launch_bounds__(geo::kBlock) stream_probe_kernel(f4v* __restri
 // arithmetic.  dst holds rows of 68 floats (17 strips, the s7 x 64 Ht pitch),
 // each updated in place from two gathered rows of src (a third as many rows,
 // like Er for Ht): src row = row / 3 + a hashed offset in [-64, 64], clamped.
-// Threads map to (row, strip) as in the step kernels and the own row carries
-// non-temporal hints.  Compulsory bytes: dst read + written, src read once.
+// Threads map to (row, strip) and blocks to XCD ranges as in the step
+// kernels, and the own row carries non-temporal hints.  Compulsory bytes: dst
+// read + written, src read once.
 __global__ void __launch_bounds__(geo::kBlock) pattern_probe_kernel(f4v* __restrict__ dst,
                                                                     const f4v* __restrict__ src,
-                                                                    int64_t rows) {
+                                                                    int64_t rows, int32_t per_xcd) {
   constexpr int kStrips = 17, kRowsPerBlock = geo::kBlock / kStrips;
   const int local_row = threadIdx.x / kStrips;
   const int strip = threadIdx.x - local_row * kStrips;
-  const int64_t row = (int64_t)blockIdx.x * kRowsPerBlock + local_row;
+  const int64_t row = (int64_t)geo::logical_block(per_xcd) * kRowsPerBlock + local_row;
   if (local_row >= kRowsPerBlock || row >= rows) return;
   const int64_t vrows = rows / 3 > 0 ? rows / 3 : 1;
   uint32_t h = (uint32_t)row * 2654435761u;
@@ -2148,16 +2149,18 @@ int geo_stream_probe(void* dst, const void* src, int64_t bytes, int32_t mode, in
   if (mode == 8) {  // edge-strip access pattern over rows of 272 bytes
     const int64_t rows = bytes / 272;
     if (rows < 3) return fail(GEO_ERR_INVALID, "pattern probe needs at least 3 rows");
-    const int64_t grid = (rows + geo::kBlock / 17 - 1) / (geo::kBlock / 17);
+    const int64_t blocks = (rows + geo::kBlock / 17 - 1) / (geo::kBlock / 17);
+    const int32_t per_xcd = (int32_t)((blocks + geo::kNumXcd - 1) / geo::kNumXcd);
+    const int64_t grid = (int64_t)per_xcd * geo::kNumXcd;
     hipEvent_t a, b;
     GEO_HIP(hipEventCreate(&a));
     GEO_HIP(hipEventCreate(&b));
     hipLaunchKernelGGL(pattern_probe_kernel, dim3((unsigned)grid), dim3(geo::kBlock), 0, s,
-                       static_cast<f4v*>(dst), static_cast<const f4v*>(src), rows);
+                       static_cast<f4v*>(dst), static_cast<const f4v*>(src), rows, per_xcd);
     GEO_HIP(hipEventRecord(a, s));
     for (int r = 0; r < repeats; ++r)
       hipLaunchKernelGGL(pattern_probe_kernel, dim3((unsigned)grid), dim3(geo::kBlock), 0, s,
-                         static_cast<f4v*>(dst), static_cast<const f4v*>(src), rows);
+                         static_cast<f4v*>(dst), static_cast<const f4v*>(src), rows, per_xcd);
     GEO_HIP(hipEventRecord(b, s));
     GEO_HIP(hipEventSynchronize(b));
     float ms = 0.f;
