@@ -183,6 +183,9 @@ typedef struct GeoLayout {
   int64_t si_pitch;        /* row pitch of the surface-impedance memory (16-byte rows) */
   int64_t plasma_basis;    /* 1 = (J.n, J.t0, J.t1) per face, n the face centre and t_c
                               its edge tangents (B = 0); 0 = Cartesian J (magnetised) */
+  int64_t radial_closed_form; /* 1 = uniform radii and FP32 arithmetic: the kernels form
+                                 1/r and the radial-derivative weights from r_0 and dr
+                                 instead of reading per-layer tables */
 } GeoLayout;
 
 typedef struct GeoDiagOut {

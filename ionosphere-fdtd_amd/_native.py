@@ -83,7 +83,8 @@ class GeoLayout(C.Structure):
                 ("band_hi", C.c_int64), ("plasma_pitch", C.c_int64),
                 ("plasma_components", C.c_int64),
                 ("vector_elems", C.c_int64), ("compute_bytes", C.c_int64),
-                ("si_pitch", C.c_int64), ("plasma_basis", C.c_int64)]
+                ("si_pitch", C.c_int64), ("plasma_basis", C.c_int64),
+                ("radial_closed_form", C.c_int64)]
 
 
 class GeoDiagOut(C.Structure):

@@ -47,6 +47,25 @@ bool valid_scale(double s) {
   return m == 0.5 && e - 1 >= -60 && e - 1 <= 60;
 }
 
+// The kernels' closed-form radial tables (FP32 compute only): used when every
+// spacing of the radii is within 1e-9 (relative) of their mean, as for the
+// default linspace altitudes, whose spacings differ in the last bits only.
+template <typename C>
+geo::RadialGrid<C> radial_grid(const std::vector<double>& rad) {
+  geo::RadialGrid<C> g{};
+  const int64_t R = (int64_t)rad.size() - 1;
+  const double dr = (rad[R] - rad[0]) / (double)R;
+  bool uniform = sizeof(C) == 4;
+  for (int64_t k = 0; k < R && uniform; ++k)
+    uniform = std::fabs((rad[k + 1] - rad[k]) - dr) <= 1e-9 * dr;
+  g.uniform = uniform ? 1 : 0;
+  g.r0_hi = (C)rad[0];
+  g.r0_lo = (C)(rad[0] - (double)g.r0_hi);
+  g.dr = (C)dr;
+  g.inv_dr = (C)(1.0 / dr);
+  return g;
+}
+
 // Owns device allocations made at plan creation.
 struct DeviceArena {
   std::vector<void*> blocks;
@@ -892,6 +911,8 @@ struct Plan final : PlanBase {
     hp.rd_b = d_rdb;
     hp.gamma_scale = full ? (C)dtm : (C)0;
     hp.inv_rbar = d_irb;
+    hp.rg = radial_grid<C>(rad);
+    hp.rb_a = (C)(dtm / ((rad[R] - rad[0]) / (double)R));
     hp.si_scale = d_sis;
     hp.si_scale_stride = (si_on && ph->si_scale_rows == E) ? 1 : 0;
     hp.si_terms = si_terms;
@@ -946,6 +967,7 @@ struct Plan final : PlanBase {
     ep.inv_rbar = d_irb;
     ep.rh_a = d_rha;
     ep.spherical = full ? 1 : 0;
+    ep.rg = hp.rg;
     ep.pl = pl;
     ep.n_src = ph->n_source;
     for (int q = 0; q < ph->n_source; ++q) {
@@ -1059,6 +1081,7 @@ struct Plan final : PlanBase {
     layout.plasma_pitch = ppitch;
     layout.plasma_components = ncomp;
     layout.plasma_basis = plasma && !magnetized ? 1 : 0;
+    layout.radial_closed_form = hp.rg.uniform;
     layout.vector_elems = N;
     layout.compute_bytes = sizeof(C);
     layout.si_pitch = si_on ? hp.si_pitch : 0;

@@ -288,6 +288,24 @@ struct PlasmaTables {
   int64_t b_row_stride;
 };
 
+// Radii r_k = r0 + k dr (the default altitude grid is a linspace): FP32
+// kernels then form 1/r_k, 1/rbar_j and the radial-derivative weights from
+// these scalars instead of reading the (pn,) / (pm,) tables, two to three
+// 16-byte table reads fewer per strip.  r0 is a head/tail pair so that
+// r_k = r0_hi + (k dr + r0_lo) rounds once.  FP64 plans keep the tables.
+template <typename C>
+struct RadialGrid {
+  int32_t uniform;
+  C r0_hi, r0_lo, dr;
+  C inv_dr;                     // 1/dr
+};
+
+// 1/r at radial position r0 + x dr (x = k for node k, j + 1/2 for midpoint j)
+template <typename C>
+__device__ __forceinline__ C inv_radius(const RadialGrid<C>& g, C x) {
+  return (C)1 / (g.r0_hi + (x * g.dr + g.r0_lo));
+}
+
 template <typename T, typename C>
 struct HParams {
   const T* __restrict__ er;
@@ -306,6 +324,8 @@ struct HParams {
   const C* __restrict__ rd_b;             // (pn,) dt/mu0 alpha_k
   C gamma_scale;                          // dt/mu0 (spherical) or 0: dt/mu0 gamma_k = gamma_scale / r_k inside
   const C* __restrict__ inv_rbar;         // (pm,)
+  RadialGrid<C> rg;
+  C rb_a;                                 // dt/(mu0 dr) on a uniform grid
   // surface impedance: memory edge-major, (E, si_pitch), each edge's terms one
   // contiguous 16-byte-aligned row read and written by the thread that owns
   // the edge's first radial strip
@@ -354,6 +374,7 @@ struct EParams {
   const C* __restrict__ inv_rbar;         // (pm,)
   const C* __restrict__ rh_a;             // (pm,) alpha'_j
   int32_t spherical;                      // gamma'_j = 1/rbar_j (spherical) or 0
+  RadialGrid<C> rg;
   PlasmaTables<C> pl;
   // sources: vertical (Er) and tangential (Et)
   int32_t n_src, n_tsrc;
@@ -392,8 +413,25 @@ __device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, const Scale
     for (int i = 0; i < N; ++i) et[i] = (C)0;
   }
   C rb[N], ir[N];
-  load_vec<N>(p.rd_b + k0, rb);
-  load_vec<N>(p.inv_r + k0, ir);
+  if (sizeof(C) == 4 && p.rg.uniform) {
+    // dt/mu0 alpha_k on r_k = r0 + k dr: dt/mu0 (1/dr - 1/(2 r_k)) inside,
+    // dt/mu0 (2/dr + 1/r_0) and dt/mu0 (2/dr - 1/r_R) at the ends on a
+    // spherical grid (gamma_scale = dt/mu0), 1/dr and 2/dr times dt/mu0 on a
+    // flat one (gamma_scale = 0); zero beyond R like the tables
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int32_t k = k0 + i;
+      const C inv = k <= p.R ? inv_radius(p.rg, (C)k) : (C)0;
+      ir[i] = inv;
+      rb[i] = k > p.R ? (C)0
+              : k == 0 ? (C)2 * p.rb_a + p.gamma_scale * inv
+              : k == p.R ? (C)2 * p.rb_a - p.gamma_scale * inv
+                         : p.rb_a - (C)0.5 * p.gamma_scale * inv;
+    }
+  } else {
+    load_vec<N>(p.rd_b + k0, rb);
+    load_vec<N>(p.inv_r + k0, ir);
+  }
   // every load is issued before the first wait (the lane shuffle below)
   C prev_loaded = (C)0;
   if (!ln.left && k0 > 0) prev_loaded = load_one(erow + k0 - 1, sc.et_inv);
@@ -441,7 +479,13 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
   T* hrow = p.hr + (size_t)f * p.pm + j0;
   load_strip_stream(hrow, hr, sc.hr_inv);
   C irb[N];
-  load_vec<N>(p.inv_rbar + j0, irb);
+  if (sizeof(C) == 4 && p.rg.uniform) {
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+      irb[i] = j0 + i < p.R ? inv_radius(p.rg, (C)(j0 + i) + (C)0.5) : (C)0;
+  } else {
+    load_vec<N>(p.inv_rbar + j0, irb);
+  }
 #pragma unroll
   for (int i = 0; i < N; ++i)  // inv_rbar is zero on padding lanes
     hr[i] = hr[i] - irb[i] * (w0 * et0[i] + w1 * et1[i] + w2 * et2[i]);
@@ -858,7 +902,12 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Sca
   load_strip(erow, er, sc.er_inv);
   load_vec<N>(p.cl_er + (size_t)v * p.ca_er_stride + k0, cl);
   load_vec<N>(p.cb_er + (size_t)v * p.ca_er_stride + k0, cb);
-  load_vec<N>(p.inv_r + k0, ir);
+  if (sizeof(C) == 4 && p.rg.uniform) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) ir[i] = k0 + i <= p.R ? inv_radius(p.rg, (C)(k0 + i)) : (C)0;
+  } else {
+    load_vec<N>(p.inv_r + k0, ir);
+  }
   C src[N];
 #pragma unroll
   for (int i = 0; i < N; ++i) src[i] = (C)0;
@@ -900,8 +949,19 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
   C cl[N], cb[N], irb[N], ra[N];
   load_vec<N>(p.cl_et + (size_t)e * p.ca_et_stride + j0, cl);
   load_vec<N>(p.cb_et + (size_t)e * p.ca_et_stride + j0, cb);
-  load_vec<N>(p.inv_rbar + j0, irb);
-  load_vec<N>(p.rh_a + j0, ra);
+  if (sizeof(C) == 4 && p.rg.uniform) {
+    // alpha'_j = r_j / (dr rbar_j) = 1/dr - 1/(2 rbar_j) (spherical), 1/dr (flat)
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      const int32_t j = j0 + i;
+      const C inv = j < p.R ? inv_radius(p.rg, (C)j + (C)0.5) : (C)0;
+      irb[i] = inv;
+      ra[i] = j < p.R ? p.rg.inv_dr - (p.spherical ? (C)0.5 * inv : (C)0) : (C)0;
+    }
+  } else {
+    load_vec<N>(p.inv_rbar + j0, irb);
+    load_vec<N>(p.rh_a + j0, ra);
+  }
   // every load (the plasma reads below included) is issued before the first
   // wait, the lane shuffle after them
   C ht_next = (C)0;

@@ -113,6 +113,40 @@ def test_fp32_within_tolerance_of_fp64_oracle(name, steps):
     _compare(sim, ref, FP32_BAR, f"{name} fp32")
 
 
+@pytest.mark.parametrize("name", ["c0_plasma", "pec_vacuum"])
+def test_fp32_closed_form_radial_tables_match_the_tables(name):
+    """On uniform radii the FP32 kernels form 1/r_k, 1/rbar_j, dt/mu0 alpha_k
+    and alpha'_j from r_0 and dr instead of reading the per-layer tables.
+    Moving one node by 1e-8 of a spacing makes the grid non-uniform to the
+    library, which then reads the tables: the two FP32 runs must agree to
+    FP32 rounding (2e-6 after 200 steps; a closed form off by a sign or half
+    a spacing changes a coefficient by dr/r ~ 5e-4), and the closed-form run
+    must stay within the FP32 bar of the FP64 oracle."""
+    sc = S.scenario(name)
+    R = sc["radial_cells"]
+    alt = np.linspace(sc["minimum_altitude_m"], sc["maximum_altitude_m"], R + 1)
+    moved = alt.copy()
+    moved[R // 2] += 1e-8 * (alt[1] - alt[0])
+    closed = S.build_sim(sc, dtype="float32")
+    tables = S.build_sim(dict(sc, radial_altitudes_m=tuple(moved)), dtype="float32")
+    assert closed._layout.radial_closed_form == 1
+    assert tables._layout.radial_closed_form == 0
+    assert S.build_sim(sc, dtype="float64")._layout.radial_closed_form == 0
+    ref = S.build_oracle(sc)
+    if name == "pec_vacuum":
+        _seed_fields(closed, ref)
+        _seed_fields(tables, S.build_oracle(sc))
+    for sim in (closed, tables, ref):
+        sim.step(200)
+    got = {k: closed.to_numpy(getattr(closed, k)) for k in FIELDS}
+    other = {k: tables.to_numpy(getattr(tables, k)) for k in FIELDS}
+    diff = S.family_errors(got, other)
+    print(name, "closed form vs tables", {k: f"{v:.2e}" for k, v in diff.items()}, flush=True)
+    for k, err in diff.items():
+        assert err <= 2e-6, f"{k}: closed form vs tables rel-L2 {err:.3e}"
+    _compare(closed, ref, FP32_BAR, f"{name} fp32 closed-form radii")
+
+
 def test_fp32_long_quasi_static_run_keeps_loss_precision():
     """C0 (20 Hz source, SI ground) for 10 000 FP32 steps within 2e-6 of FP64.
 
