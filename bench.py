@@ -416,7 +416,7 @@ def main() -> int:
 
         probe = {}
         for mode, label in ((0, "copy"), (1, "read"), (2, "write"), (4, "copy_nt"),
-                            (5, "read_nt"), (6, "write_nt"), (8, "edge_pattern")):
+                            (5, "read_nt"), (6, "write_nt"), (8, "edge_pattern"), (9, "edge_pattern_indexed")):
             probe[label] = round(_native.stream_probe(torch, 1 << 30, mode), 1)
         result["stream_probe_gbs"] = probe
     if args.copy_calibration:

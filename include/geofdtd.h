@@ -254,7 +254,10 @@ void geo_plan_destroy(GeoPlan* plan);
  * 2 write dst, +4 with non-temporal hints; mode 8 is the access pattern of
  * the H kernel's edge strips without arithmetic, in the step kernels' XCD block
  * order (rows of 68 floats in dst
- * updated in place from two gathered rows of src, a third as many rows).
+ * updated in place from two gathered rows of src, a third as many rows);
+ * mode 9 is the same with the two src rows read from an int32 index table
+ * that the probe allocates and fills, so that each gather waits on an index
+ * load as the step kernels' gathers do.
  * The measured rates are the achievable HBM ceilings reported beside the
  * roofline, and the copy's rocprofv3 FETCH_SIZE / WRITE_SIZE readings
  * calibrate the counters. */

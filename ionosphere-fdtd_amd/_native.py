@@ -194,9 +194,9 @@ def stream_probe(torch, bytes_: int, mode: int, repeats: int = 10) -> float:
     check(lib.geo_stream_probe(dst.data_ptr(), src.data_ptr(), n * 4, mode, repeats,
                                C.c_void_p(torch.cuda.current_stream().cuda_stream),
                                C.byref(out)), lib)
-    if mode == 8:  # rows of 272 B read and written, a third as many gathered rows read
+    if mode in (8, 9):  # rows of 272 B read and written, a third as many gathered rows read
         rows = (n * 4) // 272
-        moved = 2 * rows * 272 + (rows // 3) * 272
+        moved = 2 * rows * 272 + (rows // 3) * 272 + (8 * rows if mode == 9 else 0)
     else:
         moved = (2 if (mode & 3) == 0 else 1) * n * 4
     return moved / (out.value * 1e-3) / 1e9

@@ -1462,20 +1462,43 @@ __global__ void __launch_bounds__(geo::kBlock) stream_probe_kernel(f4v* __restri
 // like Er for Ht): src row = row / 3 + a hashed offset in [-64, 64], clamped.
 // Threads map to (row, strip) and blocks to XCD ranges as in the step
 // kernels, and the own row carries non-temporal hints.  Compulsory bytes: dst
-// read + written, src read once.
+// read + written, src read once.  Mode 9 (INDEXED) reads the two src rows
+// from an int32 (rows, 2) table, as the step kernels read an edge's vertices,
+// so that every gather waits on an index load first (+8 bytes per row).
+__device__ __forceinline__ void pattern_rows(int64_t row, int64_t rows, int64_t* a, int64_t* b) {
+  const int64_t vrows = rows / 3 > 0 ? rows / 3 : 1;
+  uint32_t h = (uint32_t)row * 2654435761u;
+  *a = std::min<int64_t>(vrows - 1, std::max<int64_t>(0, row / 3 + (int64_t)(h >> 25) - 64));
+  h = h * 1597334677u + 12345u;
+  *b = std::min<int64_t>(vrows - 1, std::max<int64_t>(0, row / 3 + (int64_t)(h >> 25) - 64));
+}
+
+__global__ void pattern_index_kernel(int32_t* __restrict__ idx, int64_t rows) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= rows) return;
+  int64_t a, b;
+  pattern_rows(row, rows, &a, &b);
+  idx[2 * row] = (int32_t)a;
+  idx[2 * row + 1] = (int32_t)b;
+}
+
+template <bool INDEXED>
 __global__ void __launch_bounds__(geo::kBlock) pattern_probe_kernel(f4v* __restrict__ dst,
                                                                     const f4v* __restrict__ src,
+                                                                    const int32_t* __restrict__ idx,
                                                                     int64_t rows, int32_t per_xcd) {
   constexpr int kStrips = 17, kRowsPerBlock = geo::kBlock / kStrips;
   const int local_row = threadIdx.x / kStrips;
   const int strip = threadIdx.x - local_row * kStrips;
   const int64_t row = (int64_t)geo::logical_block(per_xcd) * kRowsPerBlock + local_row;
   if (local_row >= kRowsPerBlock || row >= rows) return;
-  const int64_t vrows = rows / 3 > 0 ? rows / 3 : 1;
-  uint32_t h = (uint32_t)row * 2654435761u;
-  const int64_t a = std::min<int64_t>(vrows - 1, std::max<int64_t>(0, row / 3 + (int64_t)(h >> 25) - 64));
-  h = h * 1597334677u + 12345u;
-  const int64_t b = std::min<int64_t>(vrows - 1, std::max<int64_t>(0, row / 3 + (int64_t)(h >> 25) - 64));
+  int64_t a, b;
+  if constexpr (INDEXED) {
+    a = idx[2 * row];
+    b = idx[2 * row + 1];
+  } else {
+    pattern_rows(row, rows, &a, &b);
+  }
   const f4v x = src[a * kStrips + strip], y = src[b * kStrips + strip];
   f4v* own = dst + row * kStrips + strip;
   f4v v = __builtin_nontemporal_load(own);
@@ -2167,29 +2190,50 @@ int geo_stream_probe(void* dst, const void* src, int64_t bytes, int32_t mode, in
   if (!dst || bytes <= 0 || (bytes & 15) || repeats < 1 || !out_ms)
     return fail(GEO_ERR_INVALID, "stream probe needs a 16-byte multiple and a destination");
   if ((mode & 3) != 2 && !src) return fail(GEO_ERR_INVALID, "stream probe needs a source");
-  if (mode != 8 && (mode & ~7)) return fail(GEO_ERR_INVALID, "unknown stream probe mode");
+  if (mode != 8 && mode != 9 && (mode & ~7)) return fail(GEO_ERR_INVALID, "unknown stream probe mode");
   hipStream_t s = static_cast<hipStream_t>(stream);
-  if (mode == 8) {  // edge-strip access pattern over rows of 272 bytes
+  if (mode == 8 || mode == 9) {  // edge-strip access pattern over rows of 272 bytes
     const int64_t rows = bytes / 272;
     if (rows < 3) return fail(GEO_ERR_INVALID, "pattern probe needs at least 3 rows");
+    if (rows >= ((int64_t)1 << 31)) return fail(GEO_ERR_INVALID, "pattern probe rows exceed int32");
     const int64_t blocks = (rows + geo::kBlock / 17 - 1) / (geo::kBlock / 17);
     const int32_t per_xcd = (int32_t)((blocks + geo::kNumXcd - 1) / geo::kNumXcd);
     const int64_t grid = (int64_t)per_xcd * geo::kNumXcd;
-    hipEvent_t a, b;
-    GEO_HIP(hipEventCreate(&a));
-    GEO_HIP(hipEventCreate(&b));
-    hipLaunchKernelGGL(pattern_probe_kernel, dim3((unsigned)grid), dim3(geo::kBlock), 0, s,
-                       static_cast<f4v*>(dst), static_cast<const f4v*>(src), rows, per_xcd);
-    GEO_HIP(hipEventRecord(a, s));
-    for (int r = 0; r < repeats; ++r)
-      hipLaunchKernelGGL(pattern_probe_kernel, dim3((unsigned)grid), dim3(geo::kBlock), 0, s,
-                         static_cast<f4v*>(dst), static_cast<const f4v*>(src), rows, per_xcd);
-    GEO_HIP(hipEventRecord(b, s));
-    GEO_HIP(hipEventSynchronize(b));
+    int32_t* idx = nullptr;
+    if (mode == 9) {
+      GEO_HIP(hipMalloc(&idx, (size_t)rows * 2 * sizeof(int32_t)));
+      hipLaunchKernelGGL(pattern_index_kernel, dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s,
+                         idx, rows);
+    }
+    auto launch = [&]() {
+      if (mode == 9)
+        hipLaunchKernelGGL(pattern_probe_kernel<true>, dim3((unsigned)grid), dim3(geo::kBlock), 0, s,
+                           static_cast<f4v*>(dst), static_cast<const f4v*>(src), idx, rows, per_xcd);
+      else
+        hipLaunchKernelGGL(pattern_probe_kernel<false>, dim3((unsigned)grid), dim3(geo::kBlock), 0, s,
+                           static_cast<f4v*>(dst), static_cast<const f4v*>(src), idx, rows, per_xcd);
+    };
+    hipEvent_t a = nullptr, b = nullptr;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&b);
     float ms = 0.f;
-    GEO_HIP(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
+    if (e == hipSuccess) {
+      launch();
+      e = hipEventRecord(a, s);
+      for (int r = 0; r < repeats && e == hipSuccess; ++r) launch();
+      if (e == hipSuccess) e = hipEventRecord(b, s);
+      if (e == hipSuccess) e = hipEventSynchronize(b);
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    }
+    if (a != nullptr) (void)hipEventDestroy(a);
+    if (b != nullptr) (void)hipEventDestroy(b);
+    if (idx != nullptr) {
+      const hipError_t sync = hipStreamSynchronize(s);  // the probe launches may still read idx
+      if (e == hipSuccess) e = sync;
+      (void)hipFree(idx);
+    }
+    if (e != hipSuccess)
+      return fail(GEO_ERR_HIP, std::string("pattern probe failed: ") + hipGetErrorString(e));
     *out_ms = ms / repeats;
     return GEO_OK;
   }
