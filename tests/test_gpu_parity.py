@@ -113,10 +113,13 @@ def test_fp32_within_tolerance_of_fp64_oracle(name, steps):
     _compare(sim, ref, FP32_BAR, f"{name} fp32")
 
 
-@pytest.mark.parametrize("name", ["c0_plasma", "pec_vacuum"])
+@pytest.mark.parametrize("name", ["c0_plasma", "pec_vacuum", "thin_trapezoid", "tangential"])
 def test_fp32_closed_form_radial_tables_match_the_tables(name):
     """On uniform radii the FP32 kernels form 1/r_k, 1/rbar_j, dt/mu0 alpha_k
-    and alpha'_j from r_0 and dr instead of reading the per-layer tables.
+    and alpha'_j from r_0 and dr instead of reading the per-layer tables
+    (spherical: c0_plasma, pec_vacuum, tangential; thin shell, whose
+    coefficients drop the 1/r terms: thin_trapezoid; R = 10 and 11 leave
+    padding lanes in the last strip, which must stay zero).
     Moving one node by 1e-8 of a spacing makes the grid non-uniform to the
     library, which then reads the tables: the two FP32 runs must agree to
     FP32 rounding (2e-6 after 200 steps; a closed form off by a sign or half
@@ -144,6 +147,9 @@ def test_fp32_closed_form_radial_tables_match_the_tables(name):
     print(name, "closed form vs tables", {k: f"{v:.2e}" for k, v in diff.items()}, flush=True)
     for k, err in diff.items():
         assert err <= 2e-6, f"{k}: closed form vs tables rel-L2 {err:.3e}"
+    for k in FIELDS:
+        buf = getattr(closed, f"_{k}_buf")
+        assert bool((buf[:, getattr(closed, k).shape[1]:] == 0).all()), f"{k} padding"
     _compare(closed, ref, FP32_BAR, f"{name} fp32 closed-form radii")
 
 
