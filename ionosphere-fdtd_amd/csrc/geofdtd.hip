@@ -612,21 +612,18 @@ struct Plan final : PlanBase {
       rh_a[j] = (C)(full ? rad[j] / (dr[j] * mid[j]) : 1.0 / dr[j]);
     }
     // ---- topology and metric (solver.py:408-442, numpy_backend.py:46-114) ----
-    std::vector<int32_t> edges(2 * E), edge_lr(2 * E), face_edges(3 * F), faces(3 * F);
-    std::vector<C> edge_grad(E), edge_dual(E), face_curl(3 * F);
+    std::vector<geo::EdgeRow<C>> hedges(E), eedges(E);
+    std::vector<int32_t> face_edges(3 * F), faces(3 * F);
+    std::vector<C> face_curl(3 * F);
     for (int64_t e = 0; e < E; ++e) {
       const int64_t t = m->edges[2 * e], h = m->edges[2 * e + 1];
       const int64_t l = m->edge_left_faces[e], rr = m->edge_right_faces[e];
       if (t < 0 || t >= V || h < 0 || h >= V || l < 0 || l >= F || rr < 0 || rr >= F)
         return fail(GEO_ERR_INVALID, "edge incidence index out of range");
-      edges[2 * e] = (int32_t)t;
-      edges[2 * e + 1] = (int32_t)h;
-      edge_lr[2 * e] = (int32_t)l;
-      edge_lr[2 * e + 1] = (int32_t)rr;
       const double theta = m->primal_edge_angles[e], phi = m->dual_edge_angles[e];
       if (!(theta > 0.0) || !(phi > 0.0)) return fail(GEO_ERR_INVALID, "degenerate edge");
-      edge_grad[e] = (C)(dtm / theta);
-      edge_dual[e] = (C)(1.0 / phi);
+      hedges[e] = {(int32_t)t, (int32_t)h, (C)(dtm / theta)};
+      eedges[e] = {(int32_t)l, (int32_t)rr, (C)(1.0 / phi)};
     }
     for (int64_t f = 0; f < F; ++f) {
       const double omega = m->face_solid_angles[f];
@@ -642,10 +639,9 @@ struct Plan final : PlanBase {
             (C)(dtm * (double)m->face_edge_signs[3 * f + c] * m->primal_edge_angles[e] / omega);
       }
     }
-    // Dual-cell incidence padded to 8 slots (two 16-byte loads per table row);
-    // padding slots repeat slot 0's edge with weight 0.
-    std::vector<int32_t> vedges(8 * V, 0);
-    std::vector<C> vw(8 * V, 0);
+    // Dual-cell incidence, 6 slots per row; padding slots repeat slot 0's
+    // edge with weight 0.
+    std::vector<geo::VertexRow<C>> vrows(V);
     std::vector<int32_t> min_edge(V, (int32_t)E);
     for (int64_t v = 0; v < V; ++v) {
       const double area = m->dual_cell_solid_angles[v];
@@ -654,11 +650,14 @@ struct Plan final : PlanBase {
         const int64_t e = m->vertex_edges[v * m->max_degree + s];
         const int64_t sg = m->vertex_edge_signs[v * m->max_degree + s];
         if (e < 0 || e >= E) return fail(GEO_ERR_INVALID, "vertex incidence out of range");
-        vedges[8 * v + s] = (int32_t)e;
-        vw[8 * v + s] = (C)((double)sg * m->dual_edge_angles[e] / area);
+        vrows[v].idx[s] = (int32_t)e;
+        vrows[v].w[s] = (C)((double)sg * m->dual_edge_angles[e] / area);
         if (sg != 0) min_edge[v] = std::min(min_edge[v], (int32_t)e);
       }
-      for (int64_t s = m->max_degree; s < 8; ++s) vedges[8 * v + s] = vedges[8 * v];
+      for (int64_t s = m->max_degree; s < 6; ++s) {
+        vrows[v].idx[s] = vrows[v].idx[0];
+        vrows[v].w[s] = (C)0;
+      }
     }
     // E-kernel processing order of the dual cells: by first incident edge, so
     // the Ht columns a vertex gathers are the ones the edge segment streams at
@@ -712,8 +711,9 @@ struct Plan final : PlanBase {
     plasma = ph->n_species > 0;
     magnetized = plasma && ph->magnetized != 0;
     int64_t band_lo = 0, band_hi = 0, ppitch = 0;
-    std::vector<C> recon, center, ltan, rtan, vfw, coef, bdir, sidew;
-    std::vector<int32_t> vfaces, eslots;
+    std::vector<C> recon, center, ltan, rtan, coef, bdir, sidew;
+    std::vector<int32_t> eslots;
+    std::vector<geo::VertexRow<C>> vfaces;
     int64_t ncomp = 0;
     if (plasma) {
       if (ph->n_species > geo::kMaxSpecies) return fail(GEO_ERR_INVALID, "too many species");
@@ -795,14 +795,14 @@ struct Plan final : PlanBase {
       }
       if (ph->vf_width < 1 || ph->vf_width > 6)
         return fail(GEO_ERR_INVALID, "vertex-face width must be at most 6");
-      vfaces.assign(8 * V, 0);
-      vfw.assign(8 * V, 0);
+      // padding slots name face 0 with weight 0
+      vfaces.assign(V, geo::VertexRow<C>{});
       for (int64_t v = 0; v < V; ++v)
         for (int64_t s = 0; s < ph->vf_width; ++s) {
           const int64_t f = ph->vertex_faces[v * ph->vf_width + s];
           if (f < 0 || f >= F) return fail(GEO_ERR_INVALID, "vertex face out of range");
-          vfaces[8 * v + s] = (int32_t)f;
-          vfw[8 * v + s] = (C)ph->vertex_face_weights[v * ph->vf_width + s];
+          vfaces[v].idx[s] = (int32_t)f;
+          vfaces[v].w[s] = (C)ph->vertex_face_weights[v * ph->vf_width + s];
         }
       const int64_t rows = ph->plasma_coef_rows;
       if (!(rows == 1 || rows == F)) return fail(GEO_ERR_INVALID, "plasma rows must be 1 or F");
@@ -830,22 +830,24 @@ struct Plan final : PlanBase {
     int rc;
 #define UP(vec, ptr) \
   if ((rc = arena.upload(vec, &ptr)) != GEO_OK) return rc
-    int32_t *d_edges, *d_lr, *d_fe, *d_faces, *d_ve, *d_vo, *d_vf = nullptr;
-    C *d_eg, *d_ed, *d_fc, *d_vw, *d_ir, *d_rdb, *d_irb, *d_rha;
+    geo::EdgeRow<C> *d_hedges, *d_eedges;
+    geo::VertexRow<C> *d_vrows, *d_vf = nullptr;
+    int32_t *d_fe, *d_faces, *d_vo;
+    C *d_fc, *d_ir, *d_rdb, *d_irb, *d_rha;
     C *d_caer, *d_cber, *d_caet, *d_cbet, *d_sis = nullptr;
     C *d_recon = nullptr, *d_center = nullptr, *d_lt = nullptr, *d_rt = nullptr;
-    C *d_vfw = nullptr, *d_coef = nullptr, *d_b = nullptr;
+    C *d_coef = nullptr, *d_b = nullptr;
     int32_t* d_es = nullptr;
     C* d_sw = nullptr;
-    UP(edges, d_edges); UP(edge_lr, d_lr); UP(face_edges, d_fe); UP(faces, d_faces);
-    UP(vedges, d_ve); UP(vorder, d_vo); UP(edge_grad, d_eg); UP(edge_dual, d_ed); UP(face_curl, d_fc);
-    UP(vw, d_vw); UP(inv_r, d_ir); UP(rd_b, d_rdb); UP(inv_rbar, d_irb);
+    UP(hedges, d_hedges); UP(eedges, d_eedges); UP(face_edges, d_fe); UP(faces, d_faces);
+    UP(vrows, d_vrows); UP(vorder, d_vo); UP(face_curl, d_fc);
+    UP(inv_r, d_ir); UP(rd_b, d_rdb); UP(inv_rbar, d_irb);
     UP(rh_a, d_rha); UP(cl_er, d_caer); UP(cb_er, d_cber);
     UP(cl_et, d_caet); UP(cb_et, d_cbet);
     if (si_on) UP(si_scale, d_sis);
     if (plasma) {
       UP(recon, d_recon);
-      UP(vfaces, d_vf); UP(vfw, d_vfw); UP(coef, d_coef);
+      UP(vfaces, d_vf); UP(coef, d_coef);
       if (magnetized) {
         UP(center, d_center); UP(ltan, d_lt); UP(rtan, d_rt);
         UP(bdir, d_b);
@@ -889,7 +891,6 @@ struct Plan final : PlanBase {
       pl.left_tan = d_lt;
       pl.right_tan = d_rt;
       pl.vertex_faces = d_vf;
-      pl.vertex_face_w = d_vfw;
       pl.coef = d_coef;
       pl.coef_row_stride = ph->plasma_coef_rows == 1 ? 0 : ppitch;
       pl.coef_q_stride = ph->plasma_coef_rows * ppitch;
@@ -902,8 +903,7 @@ struct Plan final : PlanBase {
     hp.pm = (int32_t)pm;
     hp.R = (int32_t)R;
     hp.si_on = si_on ? 1 : 0;
-    hp.edges = d_edges;
-    hp.edge_grad = d_eg;
+    hp.edge_rows = d_hedges;
     hp.face_edges = d_fe;
     hp.faces = d_faces;
     hp.face_curl = d_fc;
@@ -955,10 +955,8 @@ struct Plan final : PlanBase {
     ep.ca_er_stride = er_rows == 1 ? 0 : (int32_t)pn;
     ep.ca_et_stride = et_rows == 1 ? 0 : (int32_t)pm;
     ep.vertex_order = d_vo;
-    ep.vertex_edges = d_ve;
-    ep.vertex_w = d_vw;
-    ep.edge_lr = d_lr;
-    ep.edge_dual = d_ed;
+    ep.vertex_rows = d_vrows;
+    ep.edge_rows = d_eedges;
     ep.cl_er = d_caer;
     ep.cb_er = d_cber;
     ep.cl_et = d_caet;
