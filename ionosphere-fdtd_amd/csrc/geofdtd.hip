@@ -666,6 +666,11 @@ struct Plan final : PlanBase {
     for (int64_t v = 0; v < V; ++v) vorder[v] = (int32_t)v;
     std::stable_sort(vorder.begin(), vorder.end(),
                      [&](int32_t a, int32_t b) { return min_edge[a] < min_edge[b]; });
+    {  // incidence rows in processing order (one dependent load fewer per strip)
+      std::vector<geo::VertexRow<C>> by_row(V);
+      for (int64_t r = 0; r < V; ++r) by_row[r] = vrows[vorder[r]];
+      vrows.swap(by_row);
+    }
     // ---- loss coefficients ----
     const int64_t er_rows = ph->er_coef_rows, et_rows = ph->et_coef_rows;
     if (!(er_rows == 1 || er_rows == V) || !(et_rows == 1 || et_rows == E))
@@ -804,6 +809,11 @@ struct Plan final : PlanBase {
           vfaces[v].idx[s] = (int32_t)f;
           vfaces[v].w[s] = (C)ph->vertex_face_weights[v * ph->vf_width + s];
         }
+      {
+        std::vector<geo::VertexRow<C>> by_row(V);
+        for (int64_t r = 0; r < V; ++r) by_row[r] = vfaces[vorder[r]];
+        vfaces.swap(by_row);
+      }
       const int64_t rows = ph->plasma_coef_rows;
       if (!(rows == 1 || rows == F)) return fail(GEO_ERR_INVALID, "plasma rows must be 1 or F");
       coef.assign((size_t)ph->n_species * 6 * rows * ppitch, 0);

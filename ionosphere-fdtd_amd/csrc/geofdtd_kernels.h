@@ -295,7 +295,8 @@ struct PlasmaTables {
   const C* center;              // (F, 4) unit normal, padded
   const C* left_tan;            // (E, 4) padded
   const C* right_tan;           // (E, 4) padded
-  const VertexRow<C>* vertex_faces;  // (V) faces around a vertex and their area weights
+  const VertexRow<C>* vertex_faces;  // (V) faces around each vertex and their area weights,
+                                     // in the E kernel's vertex processing order
   const C* coef;                // (species, 6, rows, pitch)
   int64_t coef_row_stride;      // 0 (uniform profile) or pitch
   int64_t coef_q_stride;        // elements between coefficient kinds
@@ -377,7 +378,8 @@ struct EParams {
   int32_t pn, pm, R;
   int32_t ca_er_stride, ca_et_stride;     // 0 or row pitch
   const int32_t* __restrict__ vertex_order;  // (V,) processing order
-  const VertexRow<C>* __restrict__ vertex_rows;  // (V) incident edges, sign*phi/A
+  const VertexRow<C>* __restrict__ vertex_rows;  // (V) incident edges, sign*phi/A, in processing
+                                                 // order (row r describes vertex_order[r])
   const EdgeRow<C>* __restrict__ edge_rows;      // (E) left, right face, 1/phi
   const C* __restrict__ cl_er;            // 1 - Ca (loss), zero on padding
   const C* __restrict__ cb_er;
@@ -815,9 +817,11 @@ template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
 __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Scales<C>& sc,
                                                int32_t row, int32_t strip, Lanes ln) {
   constexpr int N = Strip<T>::N;
+  // the incidence rows are stored in processing order, so they are loaded
+  // together with the vertex id rather than after it
   const int32_t v = p.vertex_order[row];
   const int32_t k0 = strip * N;
-  const VertexRow<C> vrow = p.vertex_rows[v];
+  const VertexRow<C> vrow = p.vertex_rows[row];
   const int32_t* ve = vrow.idx;
   const C* vw = vrow.w;
   C curl[N];
@@ -840,7 +844,7 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Sca
       C rv[N + 1];
 #pragma unroll
       for (int i = 0; i <= N; ++i) rv[i] = (C)0;
-      const VertexRow<C> frow = pl.vertex_faces[v];
+      const VertexRow<C> frow = pl.vertex_faces[row];
       const int32_t* vf = frow.idx;
       const C* vfw = frow.w;
       if (!MAGNETIZED && pl.n_species == 1) {
