@@ -486,13 +486,22 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
                                              int32_t strip, Lanes ln) {
   constexpr int N = Strip<T>::N;
   const int32_t j0 = strip * N;
+  // own row first, then the face's edges (and, in the plasma band, its
+  // vertices), so that the gathers wait on indices only
+  C hr[N];
+  T* hrow = p.hr + (size_t)f * p.pm + j0;
+  load_strip_stream(hrow, hr, sc.hr_inv);
   const int32_t e0 = p.face_edges[3 * f];
   const int32_t e1 = p.face_edges[3 * f + 1];
   const int32_t e2 = p.face_edges[3 * f + 2];
   const C w0 = p.face_curl[3 * f], w1 = p.face_curl[3 * f + 1], w2 = p.face_curl[3 * f + 2];
-  C et0[N], et1[N], et2[N], hr[N];
-  T* hrow = p.hr + (size_t)f * p.pm + j0;
-  load_strip_stream(hrow, hr, sc.hr_inv);
+  int32_t fv0 = 0, fv1 = 0, fv2 = 0;
+  if (PLASMA && j0 >= p.pl.band_lo && j0 < p.pl.band_hi) {
+    fv0 = p.faces[3 * f];
+    fv1 = p.faces[3 * f + 1];
+    fv2 = p.faces[3 * f + 2];
+  }
+  C et0[N], et1[N], et2[N];
   load_strip(p.et + (size_t)e0 * p.pm + j0, et0, sc.et_inv);
   load_strip(p.et + (size_t)e1 * p.pm + j0, et1, sc.et_inv);
   load_strip(p.et + (size_t)e2 * p.pm + j0, et2, sc.et_inv);
@@ -516,7 +525,7 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
     if (j0 < pl.band_lo || j0 >= pl.band_hi) return;
     // E at (face, midpoint): tangential reconstruction + radial node average
     // (plasma.py:492-500).
-    const int32_t v0 = p.faces[3 * f], v1 = p.faces[3 * f + 1], v2 = p.faces[3 * f + 2];
+    const int32_t v0 = fv0, v1 = fv1, v2 = fv2;
     C a0[N + 1], a1[N + 1], a2[N + 1];
     load_strip(p.er + (size_t)v0 * p.pn + j0, a0, sc.er_inv);
     load_strip(p.er + (size_t)v1 * p.pn + j0, a1, sc.er_inv);
