@@ -612,7 +612,8 @@ struct Plan final : PlanBase {
       rh_a[j] = (C)(full ? rad[j] / (dr[j] * mid[j]) : 1.0 / dr[j]);
     }
     // ---- topology and metric (solver.py:408-442, numpy_backend.py:46-114) ----
-    std::vector<geo::EdgeRow<C>> hedges(E), eedges(E);
+    std::vector<geo::EdgeRow<C>> hedges(E);
+    std::vector<geo::DualEdgeRow<C>> eedges(E);
     std::vector<int32_t> face_edges(3 * F), faces(3 * F);
     std::vector<C> face_curl(3 * F);
     for (int64_t e = 0; e < E; ++e) {
@@ -623,7 +624,7 @@ struct Plan final : PlanBase {
       const double theta = m->primal_edge_angles[e], phi = m->dual_edge_angles[e];
       if (!(theta > 0.0) || !(phi > 0.0)) return fail(GEO_ERR_INVALID, "degenerate edge");
       hedges[e] = {(int32_t)t, (int32_t)h, (C)(dtm / theta)};
-      eedges[e] = {(int32_t)l, (int32_t)rr, (C)(1.0 / phi)};
+      eedges[e] = {(int32_t)l, (int32_t)rr, (C)(1.0 / phi), 0};
     }
     for (int64_t f = 0; f < F; ++f) {
       const double omega = m->face_solid_angles[f];
@@ -774,6 +775,7 @@ struct Plan final : PlanBase {
               if (m->face_edges[3 * f + c] == e) slot = c;
             if (slot < 0) return fail(GEO_ERR_INVALID, "edge is absent from an adjacent face");
             eslots[2 * e + side] = slot;
+            eedges[e].slots |= slot << (8 * side);
           }
         sidew.assign(4 * E, 0);
         for (int64_t e = 0; e < E; ++e)
@@ -840,14 +842,14 @@ struct Plan final : PlanBase {
     int rc;
 #define UP(vec, ptr) \
   if ((rc = arena.upload(vec, &ptr)) != GEO_OK) return rc
-    geo::EdgeRow<C> *d_hedges, *d_eedges;
+    geo::EdgeRow<C>* d_hedges;
+    geo::DualEdgeRow<C>* d_eedges;
     geo::VertexRow<C> *d_vrows, *d_vf = nullptr;
     int32_t *d_fe, *d_faces, *d_vo;
     C *d_fc, *d_ir, *d_rdb, *d_irb, *d_rha;
     C *d_caer, *d_cber, *d_caet, *d_cbet, *d_sis = nullptr;
     C *d_recon = nullptr, *d_center = nullptr, *d_lt = nullptr, *d_rt = nullptr;
     C *d_coef = nullptr, *d_b = nullptr;
-    int32_t* d_es = nullptr;
     C* d_sw = nullptr;
     UP(hedges, d_hedges); UP(eedges, d_eedges); UP(face_edges, d_fe); UP(faces, d_faces);
     UP(vrows, d_vrows); UP(vorder, d_vo); UP(face_curl, d_fc);
@@ -862,7 +864,6 @@ struct Plan final : PlanBase {
         UP(center, d_center); UP(ltan, d_lt); UP(rtan, d_rt);
         UP(bdir, d_b);
       } else {
-        UP(eslots, d_es);
         UP(sidew, d_sw);
       }
     }
@@ -894,7 +895,6 @@ struct Plan final : PlanBase {
       pl.pitch = (int32_t)ppitch;
       pl.comp_stride = F * ppitch;
       pl.species_stride = ncomp * F * ppitch;
-      pl.edge_slots = d_es;
       pl.side_w = d_sw;
       pl.recon = d_recon;
       pl.center = d_center;

@@ -167,6 +167,16 @@ struct alignas(16) EdgeRow {
 };
 static_assert(sizeof(EdgeRow<float>) == 16 && sizeof(EdgeRow<double>) == 16, "one 16-byte row");
 
+// The E kernel's edge row also carries the edge's slot in its left and right
+// face (slot_l | slot_r << 8; 0 without plasma), read by the plasma scatter.
+template <typename C>
+struct alignas(16) DualEdgeRow {
+  int32_t a, b;  // left, right face
+  C w;           // 1/phi
+  int32_t slots;
+};
+static_assert(sizeof(DualEdgeRow<float>) == 16, "one 16-byte row");
+
 template <typename C>
 struct alignas(16) VertexRow {
   int32_t idx[6];  // incident edges (faces); padding slots repeat slot 0 with weight 0
@@ -290,7 +300,6 @@ struct PlasmaTables {
   // B = 0: (F, 8) = P[0][0..2], P[1][0..2] (P[c][c'] = t_c.R_c'), w1, w2;
   // magnetised: (F, 9) R = pinv(T).T
   const C* recon;
-  const int32_t* edge_slots;    // (E, 2) slot of the edge in its left / right face
   const C* side_w;              // (E, 4) B = 0: (w1, w2) left side, then right; (1, 0) for stored slots
   const C* center;              // (F, 4) unit normal, padded
   const C* left_tan;            // (E, 4) padded
@@ -380,7 +389,7 @@ struct EParams {
   const int32_t* __restrict__ vertex_order;  // (V,) processing order
   const VertexRow<C>* __restrict__ vertex_rows;  // (V) incident edges, sign*phi/A, in processing
                                                  // order (row r describes vertex_order[r])
-  const EdgeRow<C>* __restrict__ edge_rows;      // (E) left, right face, 1/phi
+  const DualEdgeRow<C>* __restrict__ edge_rows;  // (E) left, right face, 1/phi, slots
   const C* __restrict__ cl_er;            // 1 - Ca (loss), zero on padding
   const C* __restrict__ cb_er;
   const C* __restrict__ cl_et;
@@ -961,7 +970,7 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
                                              int32_t strip, Lanes ln) {
   constexpr int N = Strip<T>::N;
   const int32_t j0 = strip * N;
-  const EdgeRow<C> row = p.edge_rows[e];
+  const DualEdgeRow<C> row = p.edge_rows[e];
   const int32_t fl = row.a, fr = row.b;
   const C inv_phi = row.w;
   C hl[N], hrr[N], ht[N + 1], et[N];
@@ -1009,7 +1018,7 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
         // slot 2 is w1 J.t0 + w2 J.t1 (J.t0 read in place of the stored
         // value, J.t1 read in addition); weights (1, 0) leave a stored
         // slot's value unchanged
-        const int32_t slot_l = pl.edge_slots[2 * e], slot_r = pl.edge_slots[2 * e + 1];
+        const int32_t slot_l = row.slots & 0xff, slot_r = row.slots >> 8;
         const int32_t ql = slot_l < 2 ? 1 + slot_l : 1, qr = slot_r < 2 ? 1 + slot_r : 1;
         C xl[N], xr[N];
         C sw[4];
