@@ -416,11 +416,39 @@ struct EParams {
 // H half-step
 // ---------------------------------------------------------------------------
 
-template <typename T, typename C>
+// Value q of 16-byte piece `piece` of a per-term coefficient array held in
+// kernel parameters (uniform): a select chain over the pieces.
+template <int SV, int P, typename C>
+__device__ __forceinline__ C si_pick(const C* arr, int32_t piece, int q) {
+  C v = arr[q];
+#pragma unroll
+  for (int s = 1; s < P; ++s) v = piece == s ? arr[s * SV + q] : v;
+  return v;
+}
+
+template <typename T, typename C, bool SPREAD>
 __device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, const Scales<C>& sc, int32_t e,
                                              int32_t strip, Lanes ln) {
   constexpr int N = Strip<T>::N;
   const int32_t k0 = strip * N;
+  // Surface-impedance memory of the edge (si_pitch values, si_p pieces of 16
+  // bytes): with one strip per thread the lanes of strips 0..si_p-1 hold one
+  // piece each -- one 16-byte load and store per lane instead of si_p of each
+  // by the first strip's lane -- when those lanes share the first strip's
+  // wave; otherwise the first strip's lane updates the whole row.
+  constexpr int SV = 16 / (int)sizeof(C);
+  constexpr int kPieces = kSiFastTerms / SV;
+  const int32_t si_p = p.si_pitch / SV;
+  const int32_t lane = (int32_t)(threadIdx.x & 63), lane0 = lane - strip;
+  const bool spread = SPREAD && p.si_on && p.si_pitch <= kSiFastTerms &&
+                      si_p <= p.g_edge.row_strips && lane0 >= 0 && lane0 + si_p <= 64;
+  C mq[SV];
+#pragma unroll
+  for (int q = 0; q < SV; ++q) mq[q] = (C)0;
+  C* si_row = p.si_mem + (size_t)e * p.si_pitch;
+  if (spread && strip < si_p) load_vec_stream<SV>(si_row + SV * strip, mq);
+  C si_scale = (C)0;
+  if (spread && strip == 0) si_scale = p.si_scale[e * p.si_scale_stride];
   // The edge's own rows are requested before its vertex indices, so that
   // they are in flight while the Er gathers wait on the indices (H kernel
   // 0.205 -> 0.201 ms at s7 x 64 with the same order in the face strip).
@@ -478,8 +506,36 @@ __device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, const Scale
     ht[i] = ht[i] + (ir[i] * (a * (erh[i] - ert[i]) - g * cur) - rb[i] * (cur - prev));
     prev = cur;
   }
-  if (p.si_on && k0 == 0)  // node 0 under a surface-impedance ground replaces the PEC value
-    ht[0] = si_boundary(p, e, a * ir[0] * (erh[0] - ert[0]), h_old0, et[0]);
+  // node 0 under a surface-impedance ground replaces the PEC value
+  if (p.si_on) {
+    const C forcing = a * ir[0] * (erh[0] - ert[0]);
+    if (spread) {
+      C part = (C)0;
+#pragma unroll
+      for (int q = 0; q < SV; ++q) part += mq[q] * si_pick<SV, kPieces>(p.si_hw, strip, q);
+      C hist = part;  // pieces summed in order at the first strip
+#pragma unroll
+      for (int s = 1; s < kPieces; ++s) {
+        const C other = __shfl(part, lane0 + s);
+        if (s < si_p) hist += other;
+      }
+      hist *= si_scale;
+      const C coupling = p.si_cs * (si_scale * p.si_hw_sum);
+      const C h = h_old0 + (forcing - p.si_cs_metric * et[0] + p.si_cs * hist - coupling * h_old0) /
+                               ((C)1 + (C)0.5 * coupling);
+      const C sum = __shfl(h + h_old0, lane0);
+      if (strip < si_p) {
+#pragma unroll
+        for (int q = 0; q < SV; ++q)
+          mq[q] = mq[q] + (sum * si_pick<SV, kPieces>(p.si_drive, strip, q) -
+                           si_pick<SV, kPieces>(p.si_loss, strip, q) * mq[q]);
+        store_vec_stream<SV>(si_row + SV * strip, mq);
+      }
+      if (strip == 0) ht[0] = h;
+    } else if (k0 == 0) {
+      ht[0] = si_boundary(p, e, forcing, h_old0, et[0]);
+    }
+  }
   store_strip_stream(hrow, ht, sc.ht);
 }
 
@@ -678,9 +734,20 @@ __device__ __forceinline__ C si_boundary(const HParams<T, C>& p, int32_t e, C fo
 #pragma unroll
         for (int q = 0; q < V; ++q) m[i + q] = (C)0;
     }
+    // summed per 16-byte piece, then over the pieces in order: the order of
+    // additions of the spread update in h_edge_strip, so that both give the
+    // same bits
+    constexpr int kPieces = kSiFastTerms / V;
+    const int32_t si_p = p.si_pitch / V;
     C hist = (C)0;
 #pragma unroll
-    for (int i = 0; i < kSiFastTerms; ++i) hist += m[i] * p.si_hw[i];
+    for (int s = 0; s < kPieces; ++s) {
+      C part = (C)0;
+#pragma unroll
+      for (int q = 0; q < V; ++q) part += m[s * V + q] * p.si_hw[s * V + q];
+      if (s == 0) hist = part;
+      else if (s < si_p) hist += part;
+    }
     hist *= scale;
     const C h = h_old + (forcing - p.si_cs_metric * et0 + p.si_cs * hist - coupling * h_old) /
                             ((C)1 + (C)0.5 * coupling);
@@ -742,7 +809,8 @@ __device__ __forceinline__ void h_dispatch(const HParams<T, C>& p, const Scales<
 #pragma unroll
       for (int s = 0; s < S; ++s) {
         const int32_t strip = col + s * p.g_edge.strips;
-        if (S == 1 || strip < p.g_edge.row_strips) h_edge_strip(p, sc, row, strip, lanes_of(p.g_edge, col, strip));
+        if (S == 1 || strip < p.g_edge.row_strips)
+          h_edge_strip<T, C, S == 1>(p, sc, row, strip, lanes_of(p.g_edge, col, strip));
       }
     }
   } else if (seg == kSegHFace) {
