@@ -416,13 +416,30 @@ struct EParams {
 // H half-step
 // ---------------------------------------------------------------------------
 
+__device__ __forceinline__ float uniform_value(float x) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x)));
+}
+__device__ __forceinline__ double uniform_value(double x) {
+  const int64_t b = __double_as_longlong(x);
+  const int32_t lo = __builtin_amdgcn_readfirstlane((int32_t)(b & 0xffffffff));
+  const int32_t hi = __builtin_amdgcn_readfirstlane((int32_t)(b >> 32));
+  return __longlong_as_double(((int64_t)hi << 32) | (uint32_t)lo);
+}
+
 // Value q of 16-byte piece `piece` of a per-term coefficient array held in
 // kernel parameters (uniform): a select chain over the pieces.
 template <int SV, int P, typename C>
 __device__ __forceinline__ C si_pick(const C* arr, int32_t piece, int q) {
-  C v = arr[q];
+  // every piece's value is taken as a uniform value first (scalar
+  // registers), then the lane's piece is selected: selecting among the
+  // kernel-argument loads themselves lets the compiler turn the chain into
+  // one per-lane load from the kernel arguments at the end of the strip
+  C vals[P];
 #pragma unroll
-  for (int s = 1; s < P; ++s) v = piece == s ? arr[s * SV + q] : v;
+  for (int s = 0; s < P; ++s) vals[s] = uniform_value(arr[s * SV + q]);
+  C v = vals[0];
+#pragma unroll
+  for (int s = 1; s < P; ++s) v = piece == s ? vals[s] : v;
   return v;
 }
 
