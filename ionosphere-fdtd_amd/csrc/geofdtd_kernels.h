@@ -352,8 +352,8 @@ struct HParams {
   RadialGrid<C> rg;
   C rb_a;                                 // dt/(mu0 dr) on a uniform grid
   // surface impedance: memory edge-major, (E, si_pitch), each edge's terms one
-  // contiguous 16-byte-aligned row read and written by the thread that owns
-  // the edge's first radial strip
+  // contiguous row of 16-byte pieces, read and written by the threads of the
+  // edge's first radial strips (one piece each) or by the first strip's alone
   C* __restrict__ si_mem;
   int32_t si_pitch;
   const C* __restrict__ si_scale;         // (rows,) sqrt(mu/sigma)
