@@ -27,6 +27,9 @@
  *                        torch_backend.py:91-96): _advance_fields
  *                        (solver.py:867-989) for n steps, full chunks replayed
  *                        as one HIP graph, remainder as single steps
+ *   geo_step_prepare  <- the compile half of backend.compile_step
+ *                        (solver.py:365-375): the chunk graph is built
+ *                        before the first geo_step call that replays it
  *   geo_sample_er     <- record_er_observations' sample() (solver.py:1122-1124)
  *   geo_set_receivers <- record_er_observations argument upload
  *                        (solver.py:1097-1120)
@@ -225,6 +228,12 @@ int geo_plan_bind(GeoPlan* plan, const GeoState* state);
  * first step (record_er_observations semantics). */
 int geo_step(GeoPlan* plan, const void* currents, int64_t n_steps, int32_t chunk,
              void* trace, void* stream);
+/* Capture and instantiate the HIP graph of `chunk` steps that geo_step
+ * replays for that chunk, without running it (the compile half of
+ * backend.compile_step, solver.py:365-375, done ahead of the first call).
+ * The graph stays valid until geo_plan_bind or geo_set_receivers changes the
+ * kernel parameters. */
+int geo_step_prepare(GeoPlan* plan, int32_t chunk);
 /* Receivers for trace recording: (n,3) vertices, (n,) node layers, (n,3)
  * weights. */
 int geo_set_receivers(GeoPlan* plan, const int64_t* vertices, const int64_t* layers,

@@ -139,6 +139,7 @@ struct PlanBase {
   virtual int bind(const GeoState* s) = 0;
   virtual int step(const void* currents, int64_t n, int32_t chunk, void* trace,
                    hipStream_t stream) = 0;
+  virtual int prepare(int32_t chunk) = 0;
   virtual int set_receivers(const int64_t* v, const int64_t* l, const double* w, int32_t n) = 0;
   virtual int sample(void* row, hipStream_t stream) = 0;
   virtual int diagnostics(GeoDiagOut* out, hipStream_t stream) = 0;
@@ -1211,6 +1212,12 @@ struct Plan final : PlanBase {
     return GEO_OK;
   }
 
+  int prepare(int32_t chunk) override {
+    if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
+    if (chunk < 2) return fail(GEO_ERR_INVALID, "a step graph needs a chunk of at least 2 steps");
+    return ensure_graph(chunk);
+  }
+
   int step(const void* currents, int64_t n, int32_t chunk, void* trace,
            hipStream_t stream) override {
     if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
@@ -2107,6 +2114,11 @@ int geo_step(GeoPlan* plan, const void* currents, int64_t n_steps, int32_t chunk
              void* stream) {
   if (!plan) return fail(GEO_ERR_INVALID, "null plan");
   return plan->impl->step(currents, n_steps, chunk, trace, static_cast<hipStream_t>(stream));
+}
+
+int geo_step_prepare(GeoPlan* plan, int32_t chunk) {
+  if (!plan) return fail(GEO_ERR_INVALID, "null plan");
+  return plan->impl->prepare(chunk);
 }
 
 int geo_set_receivers(GeoPlan* plan, const int64_t* vertices, const int64_t* layers,

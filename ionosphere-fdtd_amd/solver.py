@@ -643,6 +643,15 @@ class GeodesicFDTD:
         self.steps += count
         self.time_s = self.steps * self.time_step_s
 
+    def prepare_compiled_step(self) -> None:
+        """Capture the HIP graph of ``compile_chunk_size`` steps ahead of the
+        first ``step`` that replays it (no-op when ``compile_step=False`` or
+        the chunk is a single step).  Setting new receivers or rebinding the
+        state (checkpoint load) rebuilds it on the next use."""
+        if self.compiled and self.compile_chunk_size > 1:
+            _native.check(self._lib.geo_step_prepare(self._plan, self.compile_chunk_size),
+                          self._lib)
+
     def _advance_fields(self, current_a: float = 0.0) -> None:
         """One leapfrog step with an explicit source current (reference signature)."""
         value = float(current_a)
@@ -708,9 +717,13 @@ class GeodesicFDTD:
         w3[:, :width] = w
         v3c, l3c, w3c = (np.ascontiguousarray(v3), np.ascontiguousarray(layers),
                          np.ascontiguousarray(w3))
+        key = (v3c.tobytes(), l3c.tobytes(), w3c.tobytes())
+        if key == getattr(self, "_receiver_key", None):
+            return n_rx  # same receivers: the plan (and its step graph) stay as they are
         _native.check(self._lib.geo_set_receivers(self._plan, _native.i64_ptr(v3c),
                                                   _native.i64_ptr(l3c), _native.f64_ptr(w3c),
                                                   n_rx), self._lib)
+        self._receiver_key = key
         return n_rx
 
     def record_h_observations(self, face_indices, face_radial_layers, face_weights,

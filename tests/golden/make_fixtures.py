@@ -18,7 +18,16 @@ Outputs (data only, no reference code):
   * checkpoint_<name>.npz -- a reference checkpoint (format version 4) after 60
                             steps, and checkpoint_<name>_continued.npz the
                             reference's fields after reloading it and running
-                            40 more steps.
+                            40 more steps;
+  * radial_grids.json    -- build_refined_radial_grid inputs and the reference's
+                            nodes (float.hex) or error message, for the
+                            reference's own test cases and seeded random ones.
+
+    PYTHONPATH=/root/reference/src python tests/golden/make_fixtures.py c3_seeded
+    PYTHONPATH=/root/reference/src python tests/golden/make_fixtures.py radial_grids
+    PYTHONPATH=/root/reference/src python tests/golden/make_fixtures.py mesh_hash 8
+write one trajectory, the radial-grid vectors, or the hashes of one more
+subdivision (merged into mesh_hashes.json).
 These pin the parts of the oracle that no closed form pins: integer mesh
 ordering, and the end-to-end conventions of the surface-impedance ADE, the
 plasma coupler and the source/receiver distributions.
@@ -44,6 +53,8 @@ from ionosphere_fdtd.data_artifacts import DatasetProvenance, VariableProvenance
 from ionosphere_fdtd.materials import EarthIonosphereMaterial  # noqa: E402
 from ionosphere_fdtd.mesh import build_geodesic_mesh  # noqa: E402
 from ionosphere_fdtd.plasma import ColdPlasmaSpecies, MeshPlasmaModel  # noqa: E402
+from ionosphere_fdtd.radial_grid import (RadialRefinementRegion,  # noqa: E402
+                                         build_refined_radial_grid)
 from ionosphere_fdtd.solver import GeodesicFDTD, SimulationConfig  # noqa: E402
 from ionosphere_fdtd.sources import (GaussianCurrent, TangentialGaussianCurrent,  # noqa: E402
                                      geographic_distribution)
@@ -57,7 +68,7 @@ FLOAT_FIELDS = ("vertices", "face_centers", "primal_edge_angles", "dual_edge_ang
                 "face_solid_angles", "dual_cell_solid_angles")
 STEPS = {"c0": 1000, "c0_pulse": 200, "c0_plasma": 200, "magnetized": 150,
          "thin_trapezoid": 150, "tangential": 150, "pec_vacuum": 100, "plasma_random": 120,
-         "magnetized_random": 120}
+         "magnetized_random": 120, "c3_seeded": 150}
 
 
 def canonical_sha(values) -> str:
@@ -89,7 +100,9 @@ def reference_sim(sc):
         minimum_altitude_m=sc["minimum_altitude_m"], maximum_altitude_m=sc["maximum_altitude_m"],
         earth_radius_m=sc["earth_radius_m"], courant_factor=sc["courant_factor"],
         radial_boundary_condition="surface-impedance" if sc["surface_sigma"] else "pec",
-        geometry_mode=sc["geometry_mode"], loss_integration=sc["loss_integration"])
+        geometry_mode=sc["geometry_mode"], loss_integration=sc["loss_integration"],
+        radial_altitudes_m=sc["radial_altitudes_m"],
+        radial_grid_policy="balanced-2to1" if sc["radial_altitudes_m"] else "smooth")
     material = Vacuum() if sc["vacuum"] else EarthIonosphereMaterial(**sc["material"])
     surface = ConductiveHalfSpaceSurface(sc["surface_sigma"]) if sc["surface_sigma"] else None
     mesh = build_geodesic_mesh(sc["subdivision"])
@@ -191,7 +204,63 @@ def write_checkpoint_case(name, first=60, second=40):
     print("checkpoint", name, resumed.steps, flush=True)
 
 
+def radial_grid_cases():
+    """The reference's own cases (tests/test_radial_grid.py) plus seeded random
+    domains, backgrounds and up to three overlapping regions."""
+    cases = [((0.0, 100e3, 10e3), [(60e3, 90e3, 1e3)]),
+             ((0.0, 100e3, 10e3), [(40e3, 80e3, 2e3), (60e3, 70e3, 500.0)]),
+             ((0.0, 100e3, 10e3), [(60e3, 90e3, 2e3)]),
+             ((0.0, 100e3, 2e3), [(60e3, 90e3, 500.0)]),
+             ((0.0, 100e3, 10e3), [(-1.0, 1e3, 100.0)]),
+             ((0.0, 100e3, 60e3), [])]
+    rng = np.random.default_rng(20261204)
+    for _ in range(60):
+        lo = float(rng.uniform(-50e3, 0.0))
+        hi = lo + float(rng.uniform(20e3, 200e3))
+        bg = float(rng.uniform(1e3, 30e3))
+        regions = []
+        for _ in range(int(rng.integers(0, 4))):
+            a = float(rng.uniform(lo, hi))
+            b = float(rng.uniform(a, hi))
+            if b > a:
+                regions.append((a, b, float(bg * 2.0 ** -rng.uniform(0.0, 6.0))))
+        cases.append(((lo, hi, bg), regions))
+    return cases
+
+
+def write_radial_grids():
+    out = []
+    for (lo, hi, bg), regions in radial_grid_cases():
+        entry = {"minimum_altitude_m": lo.hex(), "maximum_altitude_m": hi.hex(),
+                 "background_step_m": bg.hex(),
+                 "regions": [[a.hex(), b.hex(), c.hex()] for a, b, c in regions]}
+        try:
+            nodes = build_refined_radial_grid(
+                lo, hi, bg, tuple(RadialRefinementRegion(*r) for r in regions))
+            entry["nodes"] = [float(x).hex() for x in nodes]
+        except ValueError as exc:
+            entry["error"] = str(exc)
+        out.append(entry)
+    (HERE / "radial_grids.json").write_text(json.dumps(out, indent=0))
+    print("radial grids", len(out), flush=True)
+
+
+def write_mesh_hash(s):
+    path = HERE / "mesh_hashes.json"
+    hashes = json.loads(path.read_text())
+    m = build_geodesic_mesh(s, orientation="polar")
+    hashes[f"s{s}_polar"] = {f: canonical_sha(getattr(m, f)) for f in INT_FIELDS + FLOAT_FIELDS}
+    path.write_text(json.dumps(hashes, indent=1, sort_keys=True))
+    print("mesh hash", s, flush=True)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "radial_grids":
+        write_radial_grids()
+        return
+    if len(sys.argv) > 2 and sys.argv[1] == "mesh_hash":
+        write_mesh_hash(int(sys.argv[2]))
+        return
     if len(sys.argv) > 1 and sys.argv[1] == "checkpoints":
         for name in ("c0_pulse", "plasma_random"):
             write_checkpoint_case(name)

@@ -138,6 +138,20 @@ def scenario(name: str, **overrides) -> dict:
             # pulsed so that a short oracle run sees the source
             base["source"] = dict(base["source"], one_over_e_half_width_s="30dt",
                                   center_time_s="90dt")
+    elif name == "c3_seeded":
+        # The C3 radial grid (2:1 band over 60-90 km, 96 cells) at s2 with SI
+        # ground and the Drude plasma.  Seeded fields make every layer active
+        # from the first step, so the graded stencils, the band junctions and
+        # the plasma band all carry weight in a relative-L2 comparison; a 1 A
+        # pulsed source at 59.5 km (between the 58.6 km node and the 60 km
+        # junction node, whose control lengths differ) keeps its fields at the
+        # seeded fields' 1e-6 scale.
+        alt = refined_band_altitudes()
+        base.update(subdivision=2, radial_cells=len(alt) - 1, radial_altitudes_m=alt,
+                    seed_fields=True, receivers="c1")
+        base["plasma"] = {"h_prime_km": 74.0, "beta_per_km": 0.3}
+        base["source"] = dict(base["source"], altitude_m=59_500.0, peak_current_a=1.0,
+                              one_over_e_half_width_s="20dt", center_time_s="60dt")
     elif name in ("vacuum_s7_r64", "vacuum_s7_r80"):
         # the reference's benchmark workload: vacuum, PEC, seeded fields, no source
         base.update(subdivision=7, radial_cells=int(name[-2:]), dtype="float32",
@@ -293,9 +307,13 @@ def _midpoint_altitudes(sc):
     return 0.5 * (radii[:-1] + radii[1:]) - sc["earth_radius_m"]
 
 
-def build_oracle(sc: dict, *, dtype=None, _dt_only=False):
+def build_oracle(sc: dict, *, dtype=None, _dt_only=False, mesh=None):
+    """The oracle's OracleSim for a scenario.  ``mesh`` may be the product's
+    mesh (bit-identical to the reference's, so to the oracle's own builder):
+    at s7 and above it saves the oracle's loop-based mesh construction."""
     o = oracle_module()
-    mesh = o.build_mesh(sc["subdivision"])
+    if mesh is None:
+        mesh = o.build_mesh(sc["subdivision"])
     plasma = None
     if sc["plasma"]:
         alt = _midpoint_altitudes(sc)
