@@ -41,9 +41,10 @@ def _sha(values):
     return hashlib.sha256(np.asarray(a, dtype=dt, order="C").tobytes()).hexdigest()
 
 
-@pytest.mark.parametrize("s", range(8))
+@pytest.mark.parametrize("s", range(9))
 def test_product_mesh_is_bit_exact_with_reference(s):
-    """Integer ordering and FP64 geometry identical to the reference, s0..s7."""
+    """Integer ordering and FP64 geometry identical to the reference, s0..s8
+    (s8 is the precision-sweep mesh, 655 362 dual cells)."""
     hashes = json.loads((GOLDEN / "mesh_hashes.json").read_text())[f"s{s}_polar"]
     mesh = g.build_geodesic_mesh(s)
     for f in INT_FIELDS + FLOAT_FIELDS:
@@ -227,3 +228,66 @@ def test_radial_refinement_matches_reference_rules():
     steps = np.diff(nodes)
     assert np.isclose(steps.min(), 500.0) and np.isclose(steps.max(), 2000.0)
     assert np.max(np.maximum(steps[:-1] / steps[1:], steps[1:] / steps[:-1])) <= 2.0
+
+
+def test_radial_refinement_equals_reference_vectors():
+    """Nodes bit-equal to the reference's build_refined_radial_grid (or the
+    same error) on its own test cases and 60 seeded random domains with up to
+    three overlapping regions (tests/golden/radial_grids.json)."""
+    from ionosphere_fdtd_amd.radial_grid import RadialRefinementRegion
+
+    cases = json.loads((GOLDEN / "radial_grids.json").read_text())
+    assert len(cases) == 66
+    built = 0
+    for case in cases:
+        lo, hi, bg = (float.fromhex(case[k]) for k in
+                      ("minimum_altitude_m", "maximum_altitude_m", "background_step_m"))
+        args = (lo, hi, bg)
+        if "error" in case:
+            with pytest.raises(ValueError) as info:
+                regions = tuple(RadialRefinementRegion(*map(float.fromhex, r))
+                                for r in case["regions"])
+                g.build_refined_radial_grid(*args, regions)
+            assert str(info.value) == case["error"]
+            continue
+        regions = tuple(RadialRefinementRegion(*map(float.fromhex, r)) for r in case["regions"])
+        nodes = g.build_refined_radial_grid(*args, regions)
+        assert [x.hex() for x in nodes] == case["nodes"]
+        built += 1
+    assert built >= 60
+
+
+def test_radial_refinement_reference_cases():
+    """tests/test_radial_grid.py:12-105 of the reference, restated."""
+    from ionosphere_fdtd_amd.radial_grid import RadialRefinementRegion, validate_radial_grid
+
+    region = RadialRefinementRegion(60_000.0, 90_000.0, 1_000.0)
+    alt = g.build_refined_radial_grid(0.0, 100_000.0, 10_000.0, (region,))
+    v = validate_radial_grid(alt)
+    steps, lower, upper = np.diff(alt), np.asarray(alt[:-1]), np.asarray(alt[1:])
+    inside = (upper > region.minimum_altitude_m) & (lower < region.maximum_altitude_m)
+    assert alt[0] == 0.0 and alt[-1] == 100_000.0
+    assert v.cells == 61 and v.maximum_adjacent_step_ratio <= 2.0
+    assert np.max(steps[inside]) <= region.maximum_step_m
+
+    regions = (RadialRefinementRegion(40_000.0, 80_000.0, 2_000.0),
+               RadialRefinementRegion(60_000.0, 70_000.0, 500.0))
+    alt = g.build_refined_radial_grid(0.0, 100_000.0, 10_000.0, regions)
+    steps, lower, upper = np.diff(alt), np.asarray(alt[:-1]), np.asarray(alt[1:])
+    for r in regions:
+        inside = (upper > r.minimum_altitude_m) & (lower < r.maximum_altitude_m)
+        assert np.max(steps[inside]) <= r.maximum_step_m
+    assert validate_radial_grid(alt).maximum_adjacent_step_ratio <= 2.0
+
+    with pytest.raises(ValueError, match="adjacent-step ratio"):
+        g.SimulationConfig(subdivision=0, radial_cells=3, minimum_altitude_m=0.0,
+                           maximum_altitude_m=7_000.0,
+                           radial_altitudes_m=(0.0, 1_000.0, 2_000.0, 7_000.0),
+                           radial_grid_policy="balanced-2to1")
+    with pytest.raises(ValueError, match="outside"):
+        g.build_refined_radial_grid(0.0, 100_000.0, 10_000.0,
+                                    (RadialRefinementRegion(-1.0, 1_000.0, 100.0),))
+    with pytest.raises(ValueError, match="reversed"):
+        RadialRefinementRegion(2.0, 1.0, 1.0)
+    with pytest.raises(TypeError):
+        g.build_refined_radial_grid(0.0, 1.0, 0.1, ((0.0, 1.0, 0.1),))

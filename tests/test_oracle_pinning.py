@@ -34,7 +34,7 @@ INT_FIELDS = ("faces", "edges", "face_edges", "face_edge_signs", "edge_left_face
 FLOAT_FIELDS = ("vertices", "face_centers", "primal_edge_angles", "dual_edge_angles",
                 "face_solid_angles", "dual_cell_solid_angles")
 TRAJECTORIES = ("c0", "c0_pulse", "c0_plasma", "magnetized", "thin_trapezoid", "tangential",
-                "pec_vacuum", "plasma_random", "magnetized_random")
+                "pec_vacuum", "plasma_random", "magnetized_random", "c3_seeded")
 
 
 def _sha(values):
@@ -263,6 +263,115 @@ def test_discrete_div_b_stays_at_roundoff():
     assert np.max(scale) > 0
 
 
+# ------------------------------------------------------- graded radial grids
+GRADED = (-10_000.0, -6_000.0, -2_000.0, -1_000.0, 0.0, 4_000.0)
+
+
+def _graded_sim(geometry_mode, altitudes=GRADED, **kw):
+    return o.OracleSim(subdivision=0, radial_cells=len(altitudes) - 1,
+                       minimum_altitude_m=altitudes[0], maximum_altitude_m=altitudes[-1],
+                       radial_altitudes_m=altitudes, geometry_mode=geometry_mode,
+                       material_arrays=_vacuum_grid(0, len(altitudes) - 1), **kw)
+
+
+def _vacuum_grid(s, r):
+    return _vacuum(s, r)
+
+
+def _swap_center_distance(sim):
+    """Planted mistake: the node-centred derivative divides by the cell step
+    above the node instead of the midpoint distance D_k = rbar_k - rbar_{k-1}
+    (the two agree on a uniform grid only)."""
+    sim._center_dist = sim._steps[1:].copy()
+
+
+def _radial_adjoint(sim, weights_ht, weights_et):
+    g = np.random.default_rng(20260805)
+    sim.et[:] = g.standard_normal(sim.et.shape)
+    sim.ht[:] = g.standard_normal(sim.ht.shape)
+    left = np.sum(sim.ht * sim.radial_derivative_et() * weights_ht[None, :])
+    right = np.sum(sim.et * sim.radial_derivative_ht() * weights_et[None, :])
+    return left, right
+
+
+@pytest.mark.parametrize("plant", [None, _swap_center_distance])
+def test_graded_radial_stencils_weighted_adjoint_thin_shell(plant):
+    """test_maxwell_invariants.py:344-374 restated: on a nonuniform grid the
+    thin-shell node and midpoint derivatives are negative adjoints under the
+    control-length weights L_k and the steps dr_j; the planted D_k <-> dr_k
+    swap breaks the identity by O(1)."""
+    sim = _graded_sim("thin-shell")
+    if plant:
+        plant(sim)
+    left, right = _radial_adjoint(sim, sim.radial_node_control_lengths_m, sim.radial_steps_m)
+    if plant:
+        assert abs(left + right) > 1e-2 * max(abs(left), abs(right))
+    else:
+        assert left + right == pytest.approx(0.0, abs=2.0e-11)
+
+
+@pytest.mark.parametrize("plant", [None, _swap_center_distance])
+def test_graded_radial_stencils_weighted_adjoint_full_spherical(plant):
+    """test_maxwell_invariants.py:399-429 restated: full-spherical weights
+    r_k^2 L_k at nodes and rbar_j^2 dr_j at midpoints."""
+    sim = _graded_sim("full-spherical")
+    if plant:
+        plant(sim)
+    left, right = _radial_adjoint(sim, sim.radial_node_control_lengths_m * sim.radii_m ** 2,
+                                  sim.radial_steps_m * sim.radial_midpoints_m ** 2)
+    if plant:
+        assert abs(left + right) > 1e-2 * max(abs(left), abs(right))
+    else:
+        assert abs(left + right) <= 2.0e-15 * max(abs(left), abs(right))
+
+
+def test_graded_radial_derivative_annihilates_constants():
+    """test_maxwell_invariants.py:377-396 restated (thin shell, constant Ht)."""
+    sim = _graded_sim("thin-shell")
+    sim.ht.fill(1.0)
+    assert np.array_equal(sim.radial_derivative_ht(), np.zeros_like(sim.et))
+
+
+@pytest.mark.parametrize("altitudes", [None, (-100e3, -60e3, -40e3, -20e3, -10e3, 0.0, 10e3,
+                                              20e3, 40e3, 100e3)])
+def test_full_spherical_metric_annihilates_inverse_radius(altitudes):
+    """test_maxwell_invariants.py:432-460 restated, on the uniform grid of the
+    reference's test and on a graded one: r*Ht = 1 and rbar*Et = 1 have zero
+    interior radial derivative; the thin-shell metric does not annihilate 1/r."""
+    kw = dict(subdivision=0, radial_cells=8, minimum_altitude_m=-100e3,
+              maximum_altitude_m=100e3, material_arrays=_vacuum(0, 8))
+    if altitudes is not None:
+        kw.update(radial_cells=len(altitudes) - 1, radial_altitudes_m=altitudes,
+                  material_arrays=_vacuum(0, len(altitudes) - 1))
+    sim = o.OracleSim(geometry_mode="full-spherical", **kw)
+    sim.et[:] = 1.0 / sim.radial_midpoints_m[None, :]
+    sim.ht[:] = 1.0 / sim.radii_m[None, :]
+    # rbar * (1 / rbar) is 1 to an ulp, divided by the smallest spacing and r:
+    # 1e-27 at the reference's 25 km spacing, 1e-26 with 10 km cells; a wrong
+    # metric leaves ~1/r^2 = 2.5e-14
+    atol = 1.0e-27 if altitudes is None else 1.0e-26
+    np.testing.assert_allclose(sim.radial_derivative_et()[:, 1:-1], 0.0, atol=atol)
+    np.testing.assert_allclose(sim.radial_derivative_ht(), 0.0, atol=atol)
+    thin = o.OracleSim(geometry_mode="thin-shell", **kw)
+    thin.et[:] = 1.0 / thin.radial_midpoints_m[None, :]
+    assert np.max(np.abs(thin.radial_derivative_et()[:, 1:-1])) > 2.0e-14
+
+
+def test_graded_grid_stays_bounded_at_the_cfl_limit():
+    """test_maxwell_invariants.py:463-496 restated: courant factor 1 on the
+    abrupt graded grid, 1000 steps from random E, max |E| below twice its start."""
+    sim = _graded_sim("full-spherical", courant_factor=1.0)
+    g = np.random.default_rng(20260805)
+    sim.er[:] = g.standard_normal(sim.er.shape)
+    sim.et[:] = g.standard_normal(sim.et.shape)
+    start = max(np.max(np.abs(sim.er)), np.max(np.abs(sim.et)))
+    worst = start
+    for _ in range(1000):
+        sim.step()
+        worst = max(worst, np.max(np.abs(sim.er)), np.max(np.abs(sim.et)))
+    assert np.isfinite(worst) and worst < 2.0 * start
+
+
 # ----------------------------------------------------- reference trajectories
 def _compare_to_fixture(sim, fx, bar):
     errs = {}
@@ -326,6 +435,7 @@ def _transpose_source_layers(sim):
     ("c0_pulse", _flip_radial_sign), ("c0_pulse", _drop_si_history),
     ("plasma_random", _swap_plasma_sides), ("magnetized_random", _swap_plasma_sides),
     ("thin_trapezoid", _transpose_source_layers), ("plasma_random", _drop_vertex_average),
+    ("c3_seeded", _swap_center_distance),
 ])
 def test_planted_oracle_mistakes_are_caught(name, plant):
     sim, traces, fx = _oracle_trajectory(name, plant)
