@@ -30,6 +30,21 @@ int geo_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_
                   int64_t a_cs, const uint16_t* B, int64_t b_rs, int64_t b_cs, const float* bias,
                   float* c_f32, uint16_t* c_bf16, int64_t ldc, int32_t accumulate, void* stream);
 
+/* geo_gemm_bf16 that allocates nothing.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * kernels, split rule and results, bit for bit, as geo_gemm_bf16; the split-K
+ * partials go to an 8 MiB FP32 scratch array compiled into the library (one
+ * per device, held until the process ends), which covers every split the
+ * rule chooses -- a larger request is refused with GEO_ERR_INVALID.  The
+ * scratch is shared by all calls on a device and keeps the last call's
+ * partials (each call writes every partial it reads), so calls that overlap
+ * on one device -- on different streams or host threads -- race. */
+int geo_gemm_bf16_noalloc(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs,
+                          int64_t a_cs, const uint16_t* B, int64_t b_rs, int64_t b_cs,
+                          const float* bias, float* c_f32, uint16_t* c_bf16, int64_t ldc,
+                          int32_t accumulate, void* stream);
+
 /* out = bf16(x) for n values. */
 int geo_f32_to_bf16(int64_t n, const float* x, uint16_t* out, void* stream);
 

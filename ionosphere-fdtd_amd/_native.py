@@ -117,8 +117,9 @@ EXPORTED_SYMBOLS = (
     "geo_ensemble_tune", "geo_ensemble_step", "geo_ensemble_destroy",
     "geo_mesh_ops_create", "geo_mesh_op", "geo_array_diff", "geo_mesh_ops_destroy",
     # include/geosurrogate.h
-    "geo_gemm_bf16", "geo_f32_to_bf16", "geo_silu_forward", "geo_silu_backward",
-    "geo_column_sums", "geo_im2col_up", "geo_col2im_up", "geo_adam_step", "geo_trace_loss",
+    "geo_gemm_bf16", "geo_gemm_bf16_noalloc", "geo_f32_to_bf16", "geo_silu_forward",
+    "geo_silu_backward", "geo_column_sums", "geo_im2col_up", "geo_col2im_up", "geo_adam_step",
+    "geo_trace_loss",
 )
 
 _lib = None
@@ -166,6 +167,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     i64, i32, f32 = C.c_int64, C.c_int32, C.c_float
     lib.geo_gemm_bf16.argtypes = [i64, i64, i64, vp, i64, i64, vp, i64, i64, vp, vp, vp, i64,
                                   i32, vp]
+    lib.geo_gemm_bf16_noalloc.argtypes = list(lib.geo_gemm_bf16.argtypes)
     lib.geo_f32_to_bf16.argtypes = [i64, vp, vp, vp]
     lib.geo_silu_forward.argtypes = [i64, vp, vp, vp]
     lib.geo_silu_backward.argtypes = [i64, vp, vp, vp, vp, vp]

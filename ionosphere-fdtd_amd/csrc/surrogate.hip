@@ -10,7 +10,17 @@
 
 #include <cfloat>
 #include <cstdint>
+#include <mutex>
 #include <string>
+
+// Split-K partial sums of geo_gemm_bf16_noalloc.  The split rule of
+// gemm_bf16 keeps splits x output tiles <= 512, so the partials (splits x M x
+// N FP32 values) never exceed 512 full 64 x 64 tiles: 8 MiB.  Every device
+// the library runs on holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns on that
+// device (the name itself is a host-side stand-in, not a device address).
+constexpr long long kGemmScratchFloats = 512LL * 64 * 64;
+static __device__ __align__(256) float scratch[kGemmScratchFloats];
 
 namespace {
 
@@ -312,13 +322,38 @@ __global__ void __launch_bounds__(256) sum_partials_kernel(const float* __restri
   if (threadIdx.x == 0) *out = red[0];
 }
 
-}  // namespace
+constexpr int64_t kMaxSplitTiles = 512;
+static_assert(kGemmScratchFloats == kMaxSplitTiles * kTM * kTN, "scratch holds the largest split");
 
-extern "C" {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+float* g_scratch_ptr[kMaxDevices] = {};
 
-int geo_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs,
-                  int64_t a_cs, const uint16_t* B, int64_t b_rs, int64_t b_cs, const float* bias,
-                  float* c_f32, uint16_t* c_bf16, int64_t ldc, int32_t accumulate, void* stream) {
+// Device address of `scratch` on the current device, looked up on the first
+// call per device (no allocation: the array lives in the code object).
+int gemm_scratch_address(float** out) {
+  int dev = 0;
+  SG_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return geo_internal_fail(GEO_ERR_UNAVAILABLE, "gemm_noalloc: device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    SG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return geo_internal_fail(GEO_ERR_HIP, "gemm_noalloc: scratch symbol not found");
+    g_scratch_ptr[dev] = static_cast<float*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return GEO_OK;
+}
+
+// C = A B (+ bias).  `scratch_partial` NULL: split-K partials come from the
+// stream-ordered allocator; otherwise they go to that buffer of
+// kGemmScratchFloats values (device address of `scratch`).
+int gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs, int64_t a_cs,
+              const uint16_t* B, int64_t b_rs, int64_t b_cs, const float* bias, float* c_f32,
+              uint16_t* c_bf16, int64_t ldc, int32_t accumulate, hipStream_t s,
+              float* scratch_partial) {
   if (M < 1 || N < 1 || K < 1 || !A || !B || (!c_f32 && !c_bf16) || ldc < N)
     return geo_internal_fail(GEO_ERR_INVALID, "gemm: bad sizes or null operands");
   if (accumulate && !c_f32)
@@ -326,12 +361,12 @@ int geo_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_
   const int64_t gy = (M + kTM - 1) / kTM, gx = (N + kTN - 1) / kTN;
   if (gy > 65535 || gx > 2147483647)
     return geo_internal_fail(GEO_ERR_INVALID, "gemm: more than 65535 row tiles");
-  hipStream_t s = static_cast<hipStream_t>(stream);
   // Few output tiles over a long K (weight gradients: K = batch x length):
-  // split K so that the launch fills the chip, at most 256 splits of >= 1024.
+  // split K so that the launch fills the chip, at most 256 splits of >= 1024
+  // and at most kMaxSplitTiles (split, tile) blocks.
   int64_t splits = 1;
-  if (gx * gy < 512 && K >= 2048) {
-    splits = 512 / (gx * gy);
+  if (gx * gy < kMaxSplitTiles && K >= 2048) {
+    splits = kMaxSplitTiles / (gx * gy);
     if (splits > K / 1024) splits = K / 1024;
     if (splits > 256) splits = 256;
     if (splits < 1) splits = 1;
@@ -345,16 +380,44 @@ int geo_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_
     SG_HIP(hipGetLastError());
     return GEO_OK;
   }
-  float* partial = nullptr;
-  SG_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(float) * splits * M * N, s));
+  float* partial = scratch_partial;
+  if (scratch_partial) {
+    if (splits * M * N > kGemmScratchFloats)
+      return geo_internal_fail(GEO_ERR_INVALID,
+                               "gemm_noalloc: split-K partials exceed the 8 MiB scratch");
+  } else {
+    SG_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(float) * splits * M * N, s));
+  }
   hipLaunchKernelGGL(gemm_bf16_kernel, dim3((uint32_t)gx, (uint32_t)gy, (uint32_t)splits),
                      dim3(256), 0, s, M, N, K, A, a_rs, a_cs, B, b_rs, b_cs, (const float*)nullptr,
                      (float*)nullptr, (uint16_t*)nullptr, ldc, 0, k_chunk, partial);
   hipLaunchKernelGGL(gemm_reduce_kernel, dim3(grid_for(M * N)), dim3(256), 0, s, M, N,
                      (int32_t)splits, (const float*)partial, bias, c_f32, c_bf16, ldc, accumulate);
   SG_HIP(hipGetLastError());
-  SG_HIP(hipFreeAsync(partial, s));
+  if (!scratch_partial) SG_HIP(hipFreeAsync(partial, s));
   return GEO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int geo_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs,
+                  int64_t a_cs, const uint16_t* B, int64_t b_rs, int64_t b_cs, const float* bias,
+                  float* c_f32, uint16_t* c_bf16, int64_t ldc, int32_t accumulate, void* stream) {
+  return gemm_bf16(M, N, K, A, a_rs, a_cs, B, b_rs, b_cs, bias, c_f32, c_bf16, ldc, accumulate,
+                   static_cast<hipStream_t>(stream), nullptr);
+}
+
+int geo_gemm_bf16_noalloc(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs,
+                          int64_t a_cs, const uint16_t* B, int64_t b_rs, int64_t b_cs,
+                          const float* bias, float* c_f32, uint16_t* c_bf16, int64_t ldc,
+                          int32_t accumulate, void* stream) {
+  float* partial = nullptr;
+  const int rc = gemm_scratch_address(&partial);
+  if (rc != GEO_OK) return rc;
+  return gemm_bf16(M, N, K, A, a_rs, a_cs, B, b_rs, b_cs, bias, c_f32, c_bf16, ldc, accumulate,
+                   static_cast<hipStream_t>(stream), partial);
 }
 
 int geo_f32_to_bf16(int64_t n, const float* x, uint16_t* out, void* stream) {
