@@ -84,37 +84,81 @@ def read_traffic(kernel: str) -> float | None:
     return None
 
 
-def cpu_baseline(steps: int = 3) -> dict:
-    """The CPU oracle (NumPy restatement of the reference array algorithm,
-    FP32 fields) on the same C1 grid, a few steps after one warm-up step."""
+# The reference's published CPU rates for its own benchmark workload (vacuum,
+# PEC, seeded 1e-6 fields, s7 x 80, FP32; 32 warm-up steps, median of 3
+# repeats of 32 steps): artifacts/benchmarks/backend-scaling-eager-rtx3060.json,
+# docs/benchmarks/backend-comparison.md:268.
+PUBLISHED_S7_R80_F32_STEPS_PER_S = {"numpy": 0.8190335876595383, "torch-cpu": 0.9233466471837272}
+
+
+def _torch_cpu_module():
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("torch_cpu", REPO / "oracle" / "torch_cpu.py")
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _time_cpu_leg(sim, warmup: int, steps: int, repeats: int) -> list[float]:
+    """The reference benchmark's method (benchmarks/backend_matrix.py:144-156):
+    warm-up steps, then `repeats` timings of `steps` steps."""
+    sim.step(warmup)
+    out = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        sim.step(steps)
+        out.append(time.perf_counter() - t0)
+    return out
+
+
+def cpu_legs(workload: str, warmup: int, steps: int, repeats: int) -> dict:
+    """The oracle (NumPy restatement of the reference array path) and its
+    torch-CPU restatement (the reference's backend="torch", device="cpu"
+    path) on the workload's grid, FP32, timed on this host.  NumPy ufuncs and
+    fancy indexing run on one core; torch uses every core."""
     import numpy as np
     import scenarios as S
 
     import ionosphere_fdtd_amd as g
 
-    o = S.oracle_module()
-    sc = S.scenario("c1")
-    mesh = g.build_geodesic_mesh(7)  # bit-identical to the oracle's own builder
-    alt = S._midpoint_altitudes(sc)
-    b, species = S._plasma_arrays(sc, mesh, alt)
-    plasma = o.OraclePlasma(b, [o.OracleSpecies(q, m, n, nu) for _, q, m, n, nu in species])
-    sim = o.OracleSim(subdivision=7, radial_cells=64, minimum_altitude_m=0.0,
-                      maximum_altitude_m=100e3, earth_radius_m=sc["earth_radius_m"],
-                      material=sc["material"], surface=o.OracleSurface(sc["surface_sigma"]),
-                      plasma=plasma, mesh=mesh, dtype="float32",
-                      source=o.OracleSource(latitude_deg=S.SOURCE_LAT,
-                                            longitude_deg=S.SOURCE_LON, altitude_m=0.0,
-                                            carrier_frequency_hz=20.0))
-    sim.step(1)
-    t0 = time.perf_counter()
-    sim.step(steps)
-    elapsed = time.perf_counter() - t0
-    cells = mesh.n_vertices * 64
-    return {"value": steps * cells / elapsed / 1e6, "unit": "Mcell-updates/s",
-            "cores": 1, "kind": "port",
-            "sample": f"{steps} FP32 steps of the s7 x 64 plasma+SI workload after 1 warm-up "
-                      f"step ({elapsed:.1f} s; NumPy ufuncs single-threaded; host has "
-                      f"{os.cpu_count()} logical CPUs)"}
+    sc = S.scenario(workload, dtype="float32")
+    mesh = g.build_geodesic_mesh(sc["subdivision"])  # bit-identical to the oracle's builder
+    cells = mesh.n_vertices * sc["radial_cells"]
+    legs = {}
+    threads = os.cpu_count() or 1
+    for leg in ("numpy", "torch-cpu"):
+        ref = S.build_oracle(sc, dtype="float32", mesh=mesh)
+        if sc["seed_fields"]:
+            S.apply_seed_fields(ref, lambda k, v: getattr(ref, k).__setitem__(slice(None), v))
+        sim = ref if leg == "numpy" else _torch_cpu_module().TorchCpuSim(ref, threads=threads)
+        times = _time_cpu_leg(sim, warmup, steps, repeats)
+        median = float(np.median(times))
+        legs[leg] = {"steps_per_s": steps / median, "mcell_updates_per_s": steps * cells / median / 1e6,
+                     "cores": 1 if leg == "numpy" else threads,
+                     "seconds": [round(t, 3) for t in times]}
+        print(f"cpu {workload} {leg}: {legs[leg]}", file=sys.stderr, flush=True)
+        del sim, ref
+    return {"workload": workload, "warmup_steps": warmup, "steps": steps, "repeats": repeats,
+            "host_logical_cpus": threads, "legs": legs}
+
+
+def cpu_baseline() -> dict:
+    """Bounded CPU sample of the benchmark workload (s7 x 64, plasma + SI,
+    FP32): one warm-up step, then the median of 3 timings of 2 steps, for the
+    NumPy and torch-CPU legs.  `value` is the faster leg."""
+    res = cpu_legs("c1", warmup=1, steps=2, repeats=3)
+    best = max(res["legs"], key=lambda k: res["legs"][k]["mcell_updates_per_s"])
+    leg = res["legs"][best]
+    return {"value": round(leg["mcell_updates_per_s"], 4), "unit": "Mcell-updates/s",
+            "cores": leg["cores"], "kind": "port",
+            "sample": (f"s7 x 64 plasma+SI FP32 workload, 1 warm-up step then median of 3 x 2 "
+                       f"steps per leg (the reference benchmark's method with fewer steps); "
+                       f"faster leg: {best}; NumPy leg single-threaded ufuncs, torch-CPU leg "
+                       f"{res['host_logical_cpus']} threads"),
+            "legs": {k: {"mcell_updates_per_s": round(v["mcell_updates_per_s"], 4),
+                         "steps_per_s": round(v["steps_per_s"], 4), "cores": v["cores"]}
+                     for k, v in res["legs"].items()}}
 
 
 def precision_sweep(args) -> dict:
@@ -324,6 +368,10 @@ def main() -> int:
     ap.add_argument("--chunk", type=int, default=100)
     ap.add_argument("--kernel-steps", type=int, default=100)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-baseline-full", action="store_true",
+                    help="only time the CPU legs with the reference benchmark's own method (32 "
+                         "warm-up steps, median of 3 x 32 steps) on its published s7 x 80 "
+                         "vacuum workload, beside the published rates")
     ap.add_argument("--workload", default="c1",
                     help="c1 (default), c1_noplasma, c1_pec, c2 (64-run ensemble), c3, "
                          "c4 (surrogate + FDTD fallback), vacuum_s7_r64, vacuum_s7_r80")
@@ -349,6 +397,12 @@ def main() -> int:
 
     import scenarios as S
 
+    if args.cpu_baseline_full:
+        res = cpu_legs("vacuum_s7_r80", warmup=32, steps=32, repeats=3)
+        for leg, v in res["legs"].items():
+            v["published_steps_per_s"] = PUBLISHED_S7_R80_F32_STEPS_PER_S[leg]
+        print(json.dumps({"metric": "CPU steps/s, reference benchmark method", **res}))
+        return 0
     torch.cuda.set_device(0)
     if args.workload == "c2":
         print(json.dumps(ensemble_bench(args)))

@@ -2169,6 +2169,12 @@ int geo_ensemble_create(GeoPlan* const* plans, int32_t n_plans, GeoEnsemble** ou
     if (!plans[i]) return fail(GEO_ERR_INVALID, "null plan");
     bases[i] = plans[i]->impl.get();
   }
+  // A plan listed twice would have its state advanced twice per step by
+  // blocks of one launch (a race on its fields) and its step counter doubled.
+  std::vector<PlanBase*> sorted(bases);
+  std::sort(sorted.begin(), sorted.end());
+  if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+    return fail(GEO_ERR_INVALID, "an ensemble may hold each plan only once");
   std::unique_ptr<EnsembleBase> impl;
   int rc = GEO_ERR_INVALID;
   if (dynamic_cast<Plan<float, float>*>(bases[0]))

@@ -36,6 +36,8 @@ class EnsembleFDTD:
             raise ValueError("an ensemble needs at least one run")
         if not all(isinstance(r, GeodesicFDTD) for r in runs):
             raise TypeError("runs must be GeodesicFDTD instances")
+        if len({id(r) for r in runs}) != len(runs):
+            raise ValueError("an ensemble may hold each run only once")
         if len({r.backend.dtype_name for r in runs}) != 1:
             raise ValueError("runs must share one storage dtype")
         if len({str(r.backend.torch_device) for r in runs}) != 1:

@@ -156,7 +156,7 @@ def scenario(name: str, **overrides) -> dict:
         # the reference's benchmark workload: vacuum, PEC, seeded fields, no source
         base.update(subdivision=7, radial_cells=int(name[-2:]), dtype="float32",
                     receivers="c1", vacuum=True, surface_sigma=None, source=None,
-                    seed_fields=True)
+                    seed_fields=True, earth_radius_m=6.371e6)
     else:
         raise KeyError(name)
     base.update(overrides)

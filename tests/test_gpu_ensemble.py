@@ -133,3 +133,15 @@ def test_ensemble_rejects_mixed_runs():
     short = (rx[1][0][:2], rx[1][1][:2], rx[1][2][:2])
     with pytest.raises(ValueError, match="receivers"):
         ens.record_er_observations([rx[0], short], 5)
+    # the same run twice: refused by the Python layer and by the C ABI itself
+    with pytest.raises(ValueError, match="only once"):
+        g.EnsembleFDTD([runs[0], runs[1], runs[0]], autotune=False)
+    import ctypes as C
+
+    from ionosphere_fdtd_amd import _native
+
+    lib = _native.load_library()
+    plans = (C.c_void_p * 2)(runs[0]._plan, runs[0]._plan)
+    handle = C.c_void_p()
+    assert lib.geo_ensemble_create(plans, 2, C.byref(handle)) == -1
+    assert b"only once" in lib.geo_last_error()

@@ -444,6 +444,36 @@ def test_planted_oracle_mistakes_are_caught(name, plant):
     assert max(errs.values()) > 1e-6, errs
 
 
+# ------------------------------------------------------ torch-CPU restatement
+@pytest.mark.parametrize("name,steps", [("c0_plasma", 60), ("magnetized", 40),
+                                        ("thin_trapezoid", 40), ("c3_seeded", 30)])
+def test_torch_cpu_leg_equals_numpy_oracle(name, steps):
+    """The CPU baseline's torch leg (oracle/torch_cpu.py) is the oracle's step
+    on torch CPU tensors: FP64 fields, SI memory and plasma current within
+    1e-12 of the NumPy oracle (SI ground, Drude and magnetised plasma,
+    vertical source, thin shell, graded radii)."""
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("torch_cpu", S.ORACLE_DIR / "torch_cpu.py")
+    tc = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(tc)
+    sc = S.scenario(name)
+    ref = S.build_oracle(sc)
+    if sc["seed_fields"]:
+        S.apply_seed_fields(ref, lambda k, v: getattr(ref, k).__setitem__(slice(None), v))
+    sim = tc.TorchCpuSim(ref, threads=2)
+    sim.step(steps)
+    ref.step(steps)
+    errs = {k: S.rel_l2(getattr(sim, k).numpy(), getattr(ref, k)) for k in ("er", "et", "hr",
+                                                                             "ht")}
+    if ref.surface is not None:
+        errs["si"] = S.rel_l2(sim.si_memory.numpy(), ref.si_memory)
+    if ref.coupler is not None:
+        errs["j"] = S.rel_l2(sim.current_density[0].numpy(), ref.coupler.current_density[0])
+    bad = {k: v for k, v in errs.items() if not v <= 1e-12}
+    assert not bad, bad
+
+
 # ---------------------------------------------------------------------------
 # Trace surrogate oracle (no reference counterpart): pinned by finite
 # differences and by the adjoint / convolution identities of its layout ops.
