@@ -183,9 +183,13 @@ def precision_sweep(args) -> dict:
             sc.update(source=None, seed_fields=True)
         ref_fields = ref_traces = None
         res = {}
-        for dtype in ["float64"] + [d for d in args.sweep_dtypes.split(",") if d != "float64"]:
+        for variant in ["float64"] + [d for d in args.sweep_dtypes.split(",") if d != "float64"]:
+            # "float16-nearest" / "bfloat16-nearest": narrow storage rounded to
+            # nearest instead of stochastically
+            dtype, _, rounding = variant.partition("-")
             t0 = time.perf_counter()
-            sim = S.build_sim(sc, dtype=dtype, compile_step=True, compile_chunk_size=args.chunk)
+            sim = S.build_sim(sc, dtype=dtype, compile_step=True, compile_chunk_size=args.chunk,
+                              storage_rounding=rounding or "rotation")
             if sc["seed_fields"]:
                 S.apply_seed_fields(sim, lambda k, v: setattr(sim, k, v))
             pts = S.receiver_points(sc)
@@ -213,8 +217,8 @@ def precision_sweep(args) -> dict:
                                                      / norm)
                 if dtype in ("float16", "bfloat16"):
                     entry["final_scales"] = sim.storage_scales
-            res[dtype] = entry
-            print(f"precision {init} {dtype}: setup {setup:.1f} s, run {run:.1f} s, {entry}",
+            res[variant] = entry
+            print(f"precision {init} {variant}: setup {setup:.1f} s, run {run:.1f} s, {entry}",
                   file=sys.stderr, flush=True)
             del sim, fields
             torch.cuda.empty_cache()
@@ -386,7 +390,9 @@ def main() -> int:
     ap.add_argument("--runs", type=int, default=64, help="ensemble runs (--workload c2)")
     ap.add_argument("--queries", type=int, default=2000, help="held-out queries (--workload c4)")
     ap.add_argument("--sweep-dtypes", default="float32,float16,bfloat16",
-                    help="storage types compared with the FP64 run in --precision-sweep")
+                    help="storage types compared with the FP64 run in --precision-sweep "
+                         "(float16-nearest / bfloat16-nearest: narrow storage rounded to "
+                         "nearest instead of stochastically)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the multi-GPU halo path is out of scope for this "

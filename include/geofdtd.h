@@ -159,6 +159,14 @@ typedef struct GeoPhysicsDesc {
   /* Conductivity (S/m) with the same rows as the coefficients, for the
    * conductive-loss diagnostic; NULL means no loss. */
   const double* sigma_er; const double* sigma_et;
+  /* Narrow storage rounding of every stored value: 0 to nearest; 1 and 2
+   * stochastically (unbiased), with the threshold of each element hashed
+   * from (step, field, element) independently every step (1) or a hashed
+   * per-element phase advanced by the golden ratio each step (2, a
+   * low-discrepancy sequence).  Runs are repeatable bit for bit.  Ignored
+   * for F32/F64. */
+  int32_t storage_rounding;
+  int32_t reserved4;
 } GeoPhysicsDesc;
 
 /* Device buffers owned by the caller.  Pitches count elements. */
@@ -234,6 +242,12 @@ int geo_step(GeoPlan* plan, const void* currents, int64_t n_steps, int32_t chunk
  * The graph stays valid until geo_plan_bind or geo_set_receivers changes the
  * kernel parameters. */
 int geo_step_prepare(GeoPlan* plan, int32_t chunk);
+/* Step index of the bound state (GeodesicFDTD.steps; the plan counts the
+ * steps it advances itself) for the stochastic rounding of narrow storage:
+ * a state restored from a checkpoint at step n rounds as the run that was
+ * never interrupted.  No reference counterpart (the reference stores FP32 /
+ * FP64 only).  Enqueued on `stream`. */
+int geo_plan_set_step_index(GeoPlan* plan, int64_t step_index, void* stream);
 /* Receivers for trace recording: (n,3) vertices, (n,) node layers, (n,3)
  * weights. */
 int geo_set_receivers(GeoPlan* plan, const int64_t* vertices, const int64_t* layers,

@@ -67,6 +67,7 @@ class GeoPhysicsDesc(C.Structure):
         ("plasma_coef", _p_f64), ("plasma_coef_rows", C.c_int64),
         ("plasma_b", _p_f64), ("plasma_b_rows", C.c_int64),
         ("sigma_er", _p_f64), ("sigma_et", _p_f64),
+        ("storage_rounding", C.c_int32), ("reserved4", C.c_int32),
     ]
 
 
@@ -109,7 +110,7 @@ class GeoProfiles(C.Structure):
 
 EXPORTED_SYMBOLS = (
     "geo_plan_create", "geo_plan_layout", "geo_plan_bind", "geo_step", "geo_step_prepare",
-    "geo_set_receivers",
+    "geo_plan_set_step_index", "geo_set_receivers",
     "geo_sample_er", "geo_diagnostics", "geo_radial_profiles", "geo_time_kernels",
     "geo_plan_destroy",
     "geo_last_error", "geo_version", "geo_stream_probe", "geo_get_storage_scales",
@@ -142,6 +143,7 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.geo_plan_bind.argtypes = [vp, C.POINTER(GeoState)]
     lib.geo_step.argtypes = [vp, vp, C.c_int64, C.c_int32, vp, vp]
     lib.geo_step_prepare.argtypes = [vp, C.c_int32]
+    lib.geo_plan_set_step_index.argtypes = [vp, C.c_int64, vp]
     lib.geo_set_receivers.argtypes = [vp, _p_i64, _p_i64, _p_f64, C.c_int32]
     lib.geo_sample_er.argtypes = [vp, vp, vp]
     lib.geo_diagnostics.argtypes = [vp, C.POINTER(GeoDiagOut), vp]

@@ -134,12 +134,21 @@ int32_t geom_blocks(const geo::SegGeom& g) {
   return (int32_t)((g.n_rows + g.rows_per_block - 1) / g.rows_per_block);
 }
 
+// Step index of the stochastic rounding of narrow stores (sr_h: the next H
+// launch rounds with it), so that a run resumed at step n rounds as the
+// uninterrupted run does.
+__global__ void set_step_index_kernel(geo::Dynamic* dyn, uint32_t index) {
+  dyn->sr_h = index;
+  dyn->sr_e = index;
+}
+
 struct PlanBase {
   virtual ~PlanBase() = default;
   virtual int bind(const GeoState* s) = 0;
   virtual int step(const void* currents, int64_t n, int32_t chunk, void* trace,
                    hipStream_t stream) = 0;
   virtual int prepare(int32_t chunk) = 0;
+  virtual int set_step_index(int64_t index, hipStream_t stream) = 0;
   virtual int set_receivers(const int64_t* v, const int64_t* l, const double* w, int32_t n) = 0;
   virtual int sample(void* row, hipStream_t stream) = 0;
   virtual int diagnostics(GeoDiagOut* out, hipStream_t stream) = 0;
@@ -515,6 +524,7 @@ struct Plan final : PlanBase {
   int32_t h_grid = 0, e_grid = 0;
   geo::Dynamic* dyn = nullptr;
   bool bound = false;
+  int32_t rounding_mode = 0;  // narrow stores: 0 nearest, 1 / 2 stochastic thresholds
   // narrow storage with dynamic power-of-two scales
   bool dyn_scaling = false;
   float src_gain_er = 0.f, src_gain_et = 0.f;
@@ -699,6 +709,10 @@ struct Plan final : PlanBase {
     if (ph->scaling != 0 && ph->scaling != 1)
       return fail(GEO_ERR_INVALID, "storage scaling must be 0 (static) or 1 (dynamic)");
     dyn_scaling = narrow && ph->scaling == 1;
+    if (ph->storage_rounding < 0 || ph->storage_rounding > 2)
+      return fail(GEO_ERR_INVALID,
+                  "storage rounding must be 0 (nearest), 1 (independent) or 2 (rotation)");
+    rounding_mode = narrow ? ph->storage_rounding : 0;
     // ---- surface impedance ----
     si_terms = ph->si_terms;
     si_on = si_terms > 0;
@@ -874,6 +888,7 @@ struct Plan final : PlanBase {
     dyn = static_cast<geo::Dynamic*>(dyn_mem);
     {
       geo::Dynamic init{};
+      init.sr_mode = rounding_mode;
       const double s0[4] = {s_er, s_et, s_hr, s_ht};
       for (int f = 0; f < 4; ++f) {
         init.scale[f] = (float)s0[f];
@@ -1209,6 +1224,14 @@ struct Plan final : PlanBase {
     graph = g;
     GEO_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
     graph_chunk = chunk;
+    return GEO_OK;
+  }
+
+  int set_step_index(int64_t index, hipStream_t stream) override {
+    if (index < 0) return fail(GEO_ERR_INVALID, "step index must be non-negative");
+    hipLaunchKernelGGL(set_step_index_kernel, dim3(1), dim3(1), 0, stream, dyn,
+                       (uint32_t)(index & 0xffffffff));
+    GEO_HIP(hipGetLastError());
     return GEO_OK;
   }
 
@@ -1786,6 +1809,8 @@ struct Ensemble final : EnsembleBase {
       v.push_back({(void*)p->hp.et, sizeof(T) * p->E * p->pm});
       v.push_back({(void*)p->hp.hr, sizeof(T) * p->F * p->pm});
       if (p->si_on) v.push_back({(void*)p->hp.si_mem, sizeof(C) * p->hp.si_pitch * p->E});
+      // the rounding step index advances with the state
+      v.push_back({(void*)&p->dyn->sr_h, 2 * sizeof(uint32_t)});
       if (p->plasma)
         v.push_back({(void*)p->hp.pl.current, sizeof(C) * p->hp.pl.n_species *
                                                   p->layout.plasma_components * p->F *
@@ -2114,6 +2139,11 @@ int geo_step(GeoPlan* plan, const void* currents, int64_t n_steps, int32_t chunk
              void* stream) {
   if (!plan) return fail(GEO_ERR_INVALID, "null plan");
   return plan->impl->step(currents, n_steps, chunk, trace, static_cast<hipStream_t>(stream));
+}
+
+int geo_plan_set_step_index(GeoPlan* plan, int64_t step_index, void* stream) {
+  if (!plan) return fail(GEO_ERR_INVALID, "null plan");
+  return plan->impl->set_step_index(step_index, static_cast<hipStream_t>(stream));
 }
 
 int geo_step_prepare(GeoPlan* plan, int32_t chunk) {

@@ -63,7 +63,14 @@ struct Dynamic {
   float inv_scale[4];
   float factor[4];        // pending exact rescale of the stored values
   int32_t overflow;       // sticky: a non-finite stored value was seen
-  int32_t reserved;
+  int32_t sr_mode;        // narrow storage: 0 nearest, 1 independent, 2 rotation thresholds
+  // Step index of the stochastic rounding (narrow storage).  The H kernel
+  // rounds with sr_h and its sampling block (the only H writer of this block)
+  // copies it to sr_e; the E kernel rounds with sr_e and one of its threads
+  // writes sr_h = sr_e + 1 for the next step.  No launch reads a value that
+  // the same launch writes.
+  uint32_t sr_h;
+  uint32_t sr_e;
 };
 
 enum : int32_t { kFieldEr = 0, kFieldEt = 1, kFieldHr = 2, kFieldHt = 3 };
@@ -92,6 +99,62 @@ template <> __device__ __forceinline__ __hip_bfloat16 from_c<__hip_bfloat16>(flo
 template <typename T> __device__ __forceinline__ T from_c(double x);
 template <> __device__ __forceinline__ double from_c<double>(double x) { return x; }
 
+// Stochastic rounding of narrow stores.  A stored FP16/BF16 value that
+// rounds to nearest loses every increment smaller than half an ulp, so a
+// slowly varying field stops following its update (quasi-static regions near
+// a low-frequency source).  v_cvt_sr_{f16,bf16}_f32 round up with probability
+// equal to the dropped fraction of an ulp, read from the high bits of a
+// 32-bit threshold word u.  Two ways of drawing u, both functions of (step,
+// field, element) only, so runs are bitwise repeatable:
+//  * independent (mode 1): u = hash(step, field, element).  Each element's
+//    rounding error is a sum of independent draws and random-walks: after N
+//    steps with a steady sub-ulp increment it is ~sqrt(N p (1 - p)) ulp.
+//  * rotation (mode 2): u = hash(field, element) + step * 0x9e3779b9, a random
+//    phase per element advanced by the golden ratio each step (a Kronecker
+//    sequence).  The thresholds of one element are equidistributed with
+//    discrepancy O(log N / N), so a steady increment is followed to O(log N)
+//    ulp; the per-element phase keeps neighbouring elements uncorrelated.
+__device__ __forceinline__ uint32_t mix32(uint32_t h) {  // murmur3 finaliser
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+typedef short sr_v2s __attribute__((ext_vector_type(2)));
+
+template <typename T> __device__ __forceinline__ T round_stochastic(float x, uint32_t u);
+template <> __device__ __forceinline__ __half round_stochastic<__half>(float x, uint32_t u) {
+  const sr_v2s v = __builtin_amdgcn_cvt_sr_f16_f32(sr_v2s{0, 0}, x, u, false);
+  return __builtin_bit_cast(__half, v[0]);
+}
+template <> __device__ __forceinline__ __hip_bfloat16 round_stochastic<__hip_bfloat16>(float x,
+                                                                                      uint32_t u) {
+  const sr_v2s v = __builtin_amdgcn_cvt_sr_bf16_f32(sr_v2s{0, 0}, x, u, false);
+  return __builtin_bit_cast(__hip_bfloat16, v[0]);
+}
+
+// How a launch rounds one field's stores.  mode 0: to nearest; otherwise the
+// threshold of element `elem` is mix32(elem * K ^ key) + offset.
+struct StoreRounding {
+  uint32_t key;
+  uint32_t offset;
+  int32_t mode;
+};
+
+__device__ __forceinline__ StoreRounding store_rounding(int32_t mode, uint32_t step,
+                                                        int32_t field) {
+  const uint32_t salt = 0x632be5abu + (uint32_t)field * 0x5bd1e995u;
+  if (mode == 2) return StoreRounding{mix32(salt), step * 0x9e3779b9u, 2};
+  return StoreRounding{mix32(step * 4u + salt), 0u, mode};
+}
+
+__device__ __forceinline__ uint32_t sr_threshold(const StoreRounding& rd, uint32_t elem) {
+  return mix32(elem * 0x9e3779b1u ^ rd.key) + rd.offset;
+}
+
 // Load one strip of N values and convert to compute type (undoing the
 // power-of-two storage scale for narrow types).
 template <typename T, typename C>
@@ -105,14 +168,34 @@ __device__ __forceinline__ void load_strip(const T* __restrict__ p, C* out, C in
   }
 }
 
+// Narrow values of one strip: scaled, then rounded to nearest or
+// stochastically (element ids elem0 + i).
 template <typename T, typename C>
-__device__ __forceinline__ void store_strip(T* __restrict__ p, const C* in, C scale) {
+__device__ __forceinline__ Strip<T> narrow_strip(const C* in, C scale, StoreRounding rd,
+                                                 uint32_t elem0) {
   constexpr int N = Strip<T>::N;
   Strip<T> s;
+  if (rd.mode != 0) {
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    if constexpr (is_narrow<T>()) s.v[i] = from_c<T>(in[i] * scale);
-    else s.v[i] = from_c<T>(in[i]);
+    for (int i = 0; i < N; ++i)
+      s.v[i] = round_stochastic<T>((float)(in[i] * scale), sr_threshold(rd, elem0 + (uint32_t)i));
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) s.v[i] = from_c<T>(in[i] * scale);
+  }
+  return s;
+}
+
+template <typename T, typename C>
+__device__ __forceinline__ void store_strip(T* __restrict__ p, const C* in, C scale,
+                                            StoreRounding rd, uint32_t elem0) {
+  constexpr int N = Strip<T>::N;
+  Strip<T> s;
+  if constexpr (is_narrow<T>()) {
+    s = narrow_strip<T>(in, scale, rd, elem0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) s.v[i] = from_c<T>(in[i]);
   }
   *reinterpret_cast<Strip<T>*>(p) = s;
 }
@@ -128,12 +211,27 @@ __device__ __forceinline__ C load_one(const T* __restrict__ p, C inv_scale) {
 template <typename C>
 struct Scales {
   C er, et, hr, ht, er_inv, et_inv, hr_inv, ht_inv;
+  // rounding of the fields this launch stores (narrow storage)
+  StoreRounding rd_er, rd_et, rd_hr, rd_ht;
 };
 
-template <typename T, typename C>
+// Storage scales, and the store rounding of a step kernel: the H kernel
+// (E_SIDE false) stores Hr / Ht with step index sr_h, the E kernel Er / Et
+// with sr_e.
+template <typename T, typename C, bool E_SIDE = false>
 __device__ __forceinline__ Scales<C> load_scales(const Dynamic* dyn) {
   Scales<C> s;
+  s.rd_er = s.rd_et = s.rd_hr = s.rd_ht = StoreRounding{0u, 0u, 0};
   if constexpr (is_narrow<T>()) {
+    const int32_t mode = dyn->sr_mode;
+    const uint32_t step = E_SIDE ? dyn->sr_e : dyn->sr_h;
+    if (E_SIDE) {
+      s.rd_er = store_rounding(mode, step, kFieldEr);
+      s.rd_et = store_rounding(mode, step, kFieldEt);
+    } else {
+      s.rd_hr = store_rounding(mode, step, kFieldHr);
+      s.rd_ht = store_rounding(mode, step, kFieldHt);
+    }
     s.er = dyn->scale[kFieldEr];
     s.et = dyn->scale[kFieldEt];
     s.hr = dyn->scale[kFieldHr];
@@ -218,13 +316,15 @@ __device__ __forceinline__ void load_strip_stream(const T* __restrict__ p, C* ou
 }
 
 template <typename T, typename C>
-__device__ __forceinline__ void store_strip_stream(T* __restrict__ p, const C* in, C scale) {
+__device__ __forceinline__ void store_strip_stream(T* __restrict__ p, const C* in, C scale,
+                                                   StoreRounding rd, uint32_t elem0) {
   constexpr int N = Strip<T>::N;
   Strip<T> s;
+  if constexpr (is_narrow<T>()) {
+    s = narrow_strip<T>(in, scale, rd, elem0);
+  } else {
 #pragma unroll
-  for (int i = 0; i < N; ++i) {
-    if constexpr (is_narrow<T>()) s.v[i] = from_c<T>(in[i] * scale);
-    else s.v[i] = from_c<T>(in[i]);
+    for (int i = 0; i < N; ++i) s.v[i] = from_c<T>(in[i]);
   }
   __builtin_nontemporal_store(__builtin_bit_cast(u32x4, s), reinterpret_cast<u32x4*>(p));
 }
@@ -553,7 +653,7 @@ __device__ __forceinline__ void h_edge_strip(const HParams<T, C>& p, const Scale
       ht[0] = si_boundary(p, e, forcing, h_old0, et[0]);
     }
   }
-  store_strip_stream(hrow, ht, sc.ht);
+  store_strip_stream(hrow, ht, sc.ht, sc.rd_ht, (uint32_t)e * (uint32_t)p.pn + (uint32_t)k0);
 }
 
 template <typename C>
@@ -598,7 +698,7 @@ __device__ __forceinline__ void h_face_strip(const HParams<T, C>& p, const Scale
 #pragma unroll
   for (int i = 0; i < N; ++i)  // inv_rbar is zero on padding lanes
     hr[i] = hr[i] - irb[i] * (w0 * et0[i] + w1 * et1[i] + w2 * et2[i]);
-  store_strip_stream(hrow, hr, sc.hr);
+  store_strip_stream(hrow, hr, sc.hr, sc.rd_hr, (uint32_t)f * (uint32_t)p.pm + (uint32_t)j0);
 
   if constexpr (PLASMA) {
     const PlasmaTables<C>& pl = p.pl;
@@ -810,7 +910,10 @@ __device__ __forceinline__ void sample_block(const HParams<T, C>& p, const Scale
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) p.dyn->counter = r + 1;
+  if (threadIdx.x == 0) {
+    p.dyn->counter = r + 1;
+    if constexpr (is_narrow<T>()) p.dyn->sr_e = p.dyn->sr_h;
+  }
 }
 
 // One block of the H kernel.  With S strips per thread (a column block of
@@ -1047,7 +1150,7 @@ __device__ __forceinline__ void e_vertex_strip(const EParams<T, C>& p, const Sca
 #pragma unroll
   for (int i = 0; i < N; ++i)
     er[i] = er[i] + (cb[i] * (ir[i] * curl[i] - jr[i] - src[i]) - cl[i] * er[i]);
-  store_strip(erow, er, sc.er);
+  store_strip(erow, er, sc.er, sc.rd_er, (uint32_t)v * (uint32_t)p.pn + (uint32_t)k0);
 }
 
 template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
@@ -1154,13 +1257,18 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
     const C rd = ra[i] * (ht[i + 1] - ht[i]) + (p.spherical ? irb[i] : (C)0) * ht[i + 1];
     et[i] = et[i] + (cb[i] * (grad - rd - jt[i] - src[i]) - cl[i] * et[i]);
   }
-  store_strip(erow, et, sc.et);
+  store_strip(erow, et, sc.et, sc.rd_et, (uint32_t)e * (uint32_t)p.pm + (uint32_t)j0);
 }
 
 template <typename T, typename C, bool PLASMA, bool MAGNETIZED, int S>
 __device__ __forceinline__ void e_dispatch(const EParams<T, C>& p, const Scales<C>& sc,
                                            uint32_t seg, int32_t b) {
   int32_t row, col;
+  if constexpr (is_narrow<T>()) {
+    // the rounding step index of the next H launch: written by one thread,
+    // read by no thread of this launch
+    if (seg == kSegEVertex && b == 0 && threadIdx.x == 0) p.dyn->sr_h = p.dyn->sr_e + 1u;
+  }
   if (seg == kSegEVertex) {
     row_and_strip(p.g_vertex, b, &row, &col);
     if (row < p.g_vertex.n_rows) {
@@ -1188,7 +1296,7 @@ template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
 __global__ void __launch_bounds__(kBlock) e_step_kernel(const EParams<T, C> p) {
   const int32_t L = logical_block(p.per_xcd);
   if (L >= p.n_blocks) return;
-  const Scales<C> sc = load_scales<T, C>(p.dyn);
+  const Scales<C> sc = load_scales<T, C, true>(p.dyn);
   const uint32_t m = p.block_map[L];
   e_dispatch<T, C, PLASMA, MAGNETIZED, 1>(p, sc, m >> 24, (int32_t)(m & 0xffffffu));
 }
@@ -1233,7 +1341,7 @@ __global__ void __launch_bounds__(kBlock) e_ensemble_kernel(
   int32_t run, b;
   ens_decode(map[L], &seg, &run, &b);
   const EParams<T, C>& p = runs[run];
-  e_dispatch<T, C, PLASMA, MAGNETIZED, S>(p, load_scales<T, C>(p.dyn), seg, b);
+  e_dispatch<T, C, PLASMA, MAGNETIZED, S>(p, load_scales<T, C, true>(p.dyn), seg, b);
 }
 
 // Final trace row of every run (block r samples run r and advances its counter).

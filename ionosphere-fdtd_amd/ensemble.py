@@ -109,6 +109,8 @@ class EnsembleFDTD:
         return out
 
     def _advance(self, count: int, traces) -> None:
+        for r in self.runs:
+            r._sync_step_index(count)
         currents = self._currents(count)
         _native.check(self._lib.geo_ensemble_step(
             self._handle, None if currents is None else currents.data_ptr(), count,

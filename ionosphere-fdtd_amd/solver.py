@@ -40,6 +40,12 @@ DEFAULT_NARROW_SCALES = {"er": 1.0, "et": 1.0, "hr": 256.0, "ht": 256.0}
 _FIELD_ORDER = ("er", "et", "hr", "ht")
 
 
+# Rounding of narrow (FP16/BF16) stored values: the C ABI's storage_rounding codes.
+STORAGE_ROUNDING = {"nearest": 0, "stochastic": 1, "rotation": 2}
+# A source envelope longer than this many steps drives quasi-static fields.
+STAGNATION_STEPS = 1.0e4
+
+
 def _is_scale(value) -> bool:
     """A power of two in [2^-60, 2^60] (exact to multiply by in every storage type)."""
     try:
@@ -169,6 +175,18 @@ class GeodesicFDTD:
 
     ``er``/``ht`` live on integer radial planes, ``hr``/``et`` halfway between
     them; magnetic fields lag the electric fields by half a step.
+
+    ``dtype`` selects the field storage: ``"float32"`` (default), ``"float64"``
+    or the narrow ``"float16"`` / ``"bfloat16"``, which keep FP32 update
+    arithmetic and FP32 plasma and surface-impedance state.  Narrow storage
+    carries ~1e-3..1e-2 relative error against FP64 (FP16; BF16 ~5x more).
+    ``storage_rounding`` picks how each stored narrow value is rounded:
+    ``"rotation"`` (default; stochastic, with a hashed per-element phase
+    advanced by the golden ratio each step), ``"stochastic"`` (independent
+    hashed thresholds every step) or ``"nearest"``.  Rounding to nearest
+    stagnates once a field's per-step change drops below half an ulp, as it
+    does under slowly varying sources (the 20 Hz benchmark source: O(1)
+    errors after 5000-10000 steps); the constructor warns for that case.
     """
 
     def __init__(
@@ -187,11 +205,15 @@ class GeodesicFDTD:
         torch_threads: int | None = None,
         storage_scales: dict[str, float] | None = None,
         storage_scaling: str = "auto",
+        storage_rounding: str = "rotation",
     ) -> None:
         if (isinstance(compile_chunk_size, bool)
                 or not isinstance(compile_chunk_size, (int, np.integer))
                 or compile_chunk_size < 1):
             raise ValueError("compile_chunk_size must be a positive integer")
+        if storage_rounding not in STORAGE_ROUNDING:
+            raise ValueError("storage_rounding must be one of " + ", ".join(
+                repr(k) for k in STORAGE_ROUNDING))
         self.compile_chunk_size = int(compile_chunk_size)
         self.config = config or SimulationConfig()
         cfg = self.config
@@ -282,6 +304,10 @@ class GeodesicFDTD:
             # FP16 needs its range managed; BF16 has the exponent range of FP32
             storage_scaling = "dynamic" if self.backend.dtype_name == "float16" else "static"
         self.storage_scaling = storage_scaling
+        # Narrow storage rounds each stored value stochastically by default:
+        # to nearest, every update smaller than half an ulp of the stored
+        # value is lost, which freezes slowly varying (quasi-static) fields.
+        self.storage_rounding = storage_rounding
         self._initial_scales = scales
         self.time_s = 0.0
         self.steps = 0
@@ -289,6 +315,7 @@ class GeodesicFDTD:
         self._plan = None
         self._keepalive: list[Any] = []
         self._create_plan()
+        self._warn_if_narrow_storage_stagnates()
 
     # ------------------------------------------------------------------ setup
     def _prepare_material_coefficients(self) -> None:
@@ -391,6 +418,7 @@ class GeodesicFDTD:
             eps_er=f64(eps_er), eps_et=f64(eps_et),
             sigma_er=f64(sig_er), sigma_et=f64(sig_et),
             scaling=int(narrow and self.storage_scaling == "dynamic"),
+            storage_rounding=STORAGE_ROUNDING[self.storage_rounding] if narrow else 0,
             scale_er=self._initial_scales["er"] if narrow else 1.0,
             scale_et=self._initial_scales["et"] if narrow else 1.0,
             scale_hr=self._initial_scales["hr"] if narrow else 1.0,
@@ -629,11 +657,41 @@ class GeodesicFDTD:
         values = self.source.currents_a(times, self.time_step_s)
         return self._tensor(values, self.backend.compute_dtype)
 
+    def _warn_if_narrow_storage_stagnates(self) -> None:
+        """Round-to-nearest FP16/BF16 storage under a slowly varying source:
+        once the per-step change of a field falls below half an ulp of its
+        stored value it is lost every step, so quasi-static fields stop
+        following the source (relative errors of order one measured on the
+        20 Hz benchmark source after 5000-10000 steps)."""
+        waveform = getattr(self.source, "waveform_parameters", None)
+        if not self._narrow() or self.storage_rounding != "nearest" or waveform is None:
+            return
+        half_width, _ = waveform(self.time_step_s)
+        if half_width > STAGNATION_STEPS * self.time_step_s:
+            import warnings
+
+            warnings.warn(
+                f"{self.backend.dtype_name} storage rounded to nearest with a source envelope "
+                f"of {half_width / self.time_step_s:.3g} steps: updates below half an ulp of "
+                "the stored field are lost and quasi-static fields stagnate; use "
+                "storage_rounding='rotation' (the default)", RuntimeWarning, stacklevel=3)
+
+    def _sync_step_index(self, advancing: int) -> None:
+        """Tell the plan the step index of the bound state when it differs from
+        the steps the plan has advanced itself (a loaded checkpoint, a caller
+        setting ``steps``): the stochastic rounding of narrow storage is keyed
+        on it.  ``advancing`` steps are about to be queued."""
+        if getattr(self, "_device_steps", 0) != self.steps:
+            _native.check(self._lib.geo_plan_set_step_index(self._plan, int(self.steps),
+                                                            self._stream()), self._lib)
+        self._device_steps = self.steps + advancing
+
     def step(self, count: int = 1) -> None:
         """Advance the fields by ``count`` complete leapfrog time steps."""
         count = self._validated_count(count, "step count", minimum=0)
         if count == 0:
             return
+        self._sync_step_index(count)
         currents = self._source_currents(count)
         chunk = self.compile_chunk_size if self.compiled else 1
         _native.check(self._lib.geo_step(
@@ -656,6 +714,8 @@ class GeodesicFDTD:
         """One leapfrog step with an explicit source current (reference signature)."""
         value = float(current_a)
         currents = self._tensor(np.asarray([value]), self.backend.compute_dtype)
+        self._sync_step_index(0)
+        self._device_steps += 1  # the caller advances ``steps`` itself
         _native.check(self._lib.geo_step(self._plan, currents.data_ptr(), 1, 1, None,
                                          self._stream()), self._lib)
         self._pending = currents
@@ -677,6 +737,7 @@ class GeodesicFDTD:
         if n_rx == 0:
             self.step(steps)
             return self.to_numpy(traces)
+        self._sync_step_index(steps)
         currents = self._source_currents(steps)
         chunk = self.compile_chunk_size if self.compiled else 1
         _native.check(self._lib.geo_step(
@@ -880,6 +941,7 @@ class GeodesicFDTD:
 
         The timed steps advance the state with a zero source current."""
         out = (C.c_double * 2)()
+        self._sync_step_index(steps)
         _native.check(self._lib.geo_time_kernels(self._plan, steps, self._stream(), out),
                       self._lib)
         self.steps += steps

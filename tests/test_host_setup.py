@@ -291,3 +291,31 @@ def test_radial_refinement_reference_cases():
         RadialRefinementRegion(2.0, 1.0, 1.0)
     with pytest.raises(TypeError):
         g.build_refined_radial_grid(0.0, 1.0, 0.1, ((0.0, 1.0, 0.1),))
+
+
+def test_nearest_narrow_storage_with_slow_source_warns(monkeypatch):
+    """Round-to-nearest narrow storage under a source whose envelope spans
+    more than 1e4 steps warns (the stagnation check runs on the host only)."""
+    from ionosphere_fdtd_amd import solver
+
+    class Probe:
+        _narrow = solver.GeodesicFDTD._narrow
+        _warn_if_narrow_storage_stagnates = solver.GeodesicFDTD._warn_if_narrow_storage_stagnates
+
+    for rounding, source, expect in (("nearest", g.GaussianCurrent(carrier_frequency_hz=20.0), True),
+                                     ("rotation", g.GaussianCurrent(carrier_frequency_hz=20.0), False),
+                                     ("nearest", g.GaussianCurrent(one_over_e_half_width_s=1e-6), False)):
+        probe = Probe()
+        probe.storage_rounding = rounding
+        probe.source = source
+        probe.time_step_s = 1.27e-7
+        probe.backend = type("B", (), {"dtype_name": "float16"})()
+        import warnings
+
+        with warnings.catch_warnings(record=True) as caught:
+            warnings.simplefilter("always")
+            probe._warn_if_narrow_storage_stagnates()
+        assert any("stagnate" in str(w.message) for w in caught) == expect, (rounding, source)
+    with pytest.raises(ValueError, match="storage_rounding"):
+        g.GeodesicFDTD(g.SimulationConfig(subdivision=0, radial_cells=2), dtype="float16",
+                       storage_rounding="up")
