@@ -301,43 +301,63 @@ def ensemble_bench(args) -> dict:
 
 
 def surrogate_bench(args) -> dict:
-    """Learned surrogate with FDTD fallback on held-out trace queries.
+    """Learned surrogate with FDTD fallback on held-out random-geometry queries.
 
-    2000 held-out queries (250 seeded C2-style configurations x 8 stations)
-    are answered (a) by batched plain FDTD -- the ground truth and the time
-    to beat -- and (b) by the query service: the trained network for
-    in-range queries, one spot-checked configuration simulated per batch
-    (any error > 2 % sends the batch to FDTD), FDTD for out-of-range queries.
-    A further 16 out-of-range queries exercise the fallback."""
+    --queries held-out queries (one receiver per configuration by default:
+    every query its own physics, its source anywhere on the sphere, its
+    receiver within 250 km; seeds disjoint from training) are answered (a) by
+    batched plain FDTD -- the ground truth and the time to beat -- and (b) by
+    the query service: the trained committee for in-range queries it agrees
+    on, FDTD for the rest, one spot-checked configuration per batch (any error
+    > 2 % sends the batch to FDTD).  A further 16 out-of-range queries
+    exercise the fallback."""
     import numpy as np
     import torch
 
     import ionosphere_fdtd_amd as g
     from ionosphere_fdtd_amd import trace_queries as tq
 
-    model_path = REPO / "ionosphere-fdtd_amd" / "data" / "trace_surrogate.npz"
-    model = tq.SurrogateModel.load(model_path)
+    paths = sorted((REPO / "ionosphere-fdtd_amd" / "data").glob("trace_surrogate_*.npz"))
+    committee = tq.SurrogateCommittee.load(paths)
     mesh = g.build_geodesic_mesh(tq.SUBDIVISION)
-    runs = tq.sample_runs(args.queries // len(tq.STATION_OFFSETS), seed=20261204)
-    queries = tq.station_queries(runs)
+    per = max(1, args.receivers_per_config)
+    queries = tq.random_queries(args.queries // per, per, seed=20261206)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     truth = tq.fdtd_traces(queries, mesh)
     torch.cuda.synchronize()
     t_fdtd = time.perf_counter() - t0
     print(f"batched FDTD: {len(queries)} queries in {t_fdtd:.1f} s", file=sys.stderr, flush=True)
-    service = tq.TraceQueryService(model, mesh, tolerance=0.02, spot_checks=1, seed=7)
-    service.answer(queries[:8])  # warm-up (graphs, kernels)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    answers, served, report = service.answer(queries)
-    torch.cuda.synchronize()
-    t_service = time.perf_counter() - t1
+    thresholds = [float(x) for x in str(args.disagreement).split(",")]
+    runs = {}
+    for tau in thresholds:
+        service = tq.TraceQueryService(committee, mesh, tolerance=0.02, disagreement=tau,
+                                       spot_checks=1, seed=7)
+        service.answer(queries[:8])  # warm-up (graphs, kernels)
+        tq._RECEIVER_TABLES.clear()  # every timed run finds its faces afresh
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        answers, served, report = service.answer(queries)
+        torch.cuda.synchronize()
+        runs[tau] = (time.perf_counter() - t1, answers, served, report)
+        e = tq.trace_rel_l2(answers, truth)
+        print(f"threshold {tau}: {runs[tau][0]:.3f} s, {report.get('uncertain')} to FDTD, "
+              f"within 2 % {np.mean(e <= 0.02):.4f}, max {e.max():.4f}", file=sys.stderr,
+              flush=True)
+    tau = thresholds[0]
+    t_service, answers, served, report = runs[tau]
     err = tq.trace_rel_l2(answers, truth)
-    # fallback: out-of-range physics and a receiver off the station network
+    pred, spread = committee.predict(queries, mesh)
+    raw = tq.trace_rel_l2(pred, truth)
+    by_net = served == "surrogate"
+    # fallback: out-of-range physics and a receiver beyond 250 km
+    from ionosphere_fdtd_amd.sources import destination_point
+
     odd = queries[:16].copy()
     odd[:8, 1] = 0.7
-    odd[8:, 5] += 0.5
+    for k in range(8, 16):
+        odd[k, 5], odd[k, 6] = destination_point(odd[k, 3], odd[k, 4], 45.0 * k, 400e3,
+                                                 tq.EARTH_RADIUS_M)
     o_ans, o_served, o_report = service.answer(odd)
     o_truth = tq.fdtd_traces(odd, mesh)
     return {
@@ -345,22 +365,47 @@ def surrogate_bench(args) -> dict:
         "value": round(len(queries) / t_service, 1), "unit": "queries/s",
         "n_gpus": 1, "higher_is_better": True, "dtype": "bf16 network / f32 FDTD",
         "data": "synthetic", "vs_baseline": None,
-        "config": {"workload": "2000 held-out C2-style station queries (s5, R=32, 4096 steps, "
-                               "512-sample Er traces)", "model": str(model_path.name),
-                   "train": "8192 runs x 8 stations, 10000 Adam steps"},
+        "config": {"workload": f"{len(queries)} held-out random-geometry queries "
+                               f"({len(queries) // per} configurations x {per} receivers; s5, "
+                               "R=32, 4096 steps, 512-sample Er traces)",
+                   "models": [p.name for p in paths],
+                   "disagreement_threshold": tau},
+        "other_thresholds": {
+            f"{t:g}": {"queries_per_s": round(len(queries) / r[0], 1),
+                       "speedup_vs_batched_fdtd": round(t_fdtd / r[0], 1),
+                       "to_fdtd": r[3].get("uncertain"),
+                       "within_2pct": float(np.mean(tq.trace_rel_l2(r[1], truth) <= 0.02)),
+                       "max_error": float(tq.trace_rel_l2(r[1], truth).max())}
+            for t, r in runs.items() if t != tau},
         "batched_fdtd_queries_per_s": round(len(queries) / t_fdtd, 2),
         "speedup_vs_batched_fdtd": round(t_fdtd / t_service, 1),
-        "served_by_surrogate": int(np.sum(served == "surrogate")),
-        "served_by_fdtd": int(np.sum(served == "fdtd")),
+        "served_by_surrogate": int(np.sum(by_net)),
+        "served_by_fdtd": int(np.sum(~by_net)),
+        "uncertain_to_fdtd": report.get("uncertain"),
         "spot_check_errors": report.get("spot_check_errors"),
         "service_seconds": {"total": round(t_service, 4),
                             "surrogate": round(report.get("surrogate_seconds", 0.0), 4),
-                            "spot_check_fdtd": round(report.get("spot_check_seconds", 0.0), 4)},
+                            "spot_check_fdtd": round(report.get("spot_check_seconds", 0.0), 4),
+                            "fallback_fdtd": round(report.get("fallback_fdtd_seconds", 0.0), 4)},
         "error_vs_fdtd": {"median": float(np.median(err)), "p90": float(np.quantile(err, 0.9)),
                           "p99": float(np.quantile(err, 0.99)), "max": float(err.max()),
                           "within_2pct": float(np.mean(err <= 0.02))},
+        "committee_alone": {"median": float(np.median(raw)), "p90": float(np.quantile(raw, 0.9)),
+                            "p99": float(np.quantile(raw, 0.99)), "max": float(raw.max()),
+                            "within_2pct": float(np.mean(raw <= 0.02)),
+                            "flagged": int(np.sum(spread > tau)),
+                            "above_2pct_not_flagged": int(np.sum((raw > 0.02)
+                                                                 & (spread <= tau)))},
         "fallback_check": {"queries": len(odd), "served_by_fdtd": int(np.sum(o_served == "fdtd")),
                            "max_error_vs_fdtd": float(tq.trace_rel_l2(o_ans, o_truth).max())},
+        # the router's trade-off: queries sent to FDTD and the errors left among
+        # the rest, for a range of disagreement thresholds
+        "threshold_sweep": {f"{t:g}": {"to_fdtd": int(np.sum(spread > t)),
+                                       "served_max_error": float(raw[spread <= t].max())
+                                       if np.any(spread <= t) else None,
+                                       "served_above_2pct": int(np.sum((raw > 0.02)
+                                                                       & (spread <= t)))}
+                            for t in (0.002, 0.004, 0.006, 0.008, 0.01, 0.015, 0.02, 0.03)},
     }
 
 
@@ -389,6 +434,12 @@ def main() -> int:
                          "bench")
     ap.add_argument("--runs", type=int, default=64, help="ensemble runs (--workload c2)")
     ap.add_argument("--queries", type=int, default=2000, help="held-out queries (--workload c4)")
+    ap.add_argument("--receivers-per-config", type=int, default=1,
+                    help="queries sharing one configuration (--workload c4)")
+    ap.add_argument("--disagreement", default="0.01",
+                    help="committee disagreement above which a query goes to FDTD "
+                         "(--workload c4); a comma-separated list times the service at "
+                         "each, the first one reported as the result")
     ap.add_argument("--sweep-dtypes", default="float32,float16,bfloat16",
                     help="storage types compared with the FP64 run in --precision-sweep "
                          "(float16-nearest / bfloat16-nearest: narrow storage rounded to "

@@ -51,18 +51,48 @@ def destination_point(latitude_deg: float, longitude_deg: float, azimuth_deg: fl
     return lat, lon
 
 
+_FACE_PLANES: dict = {}
+_NEAREST_FACES = 24
+
+
+def _face_planes(mesh):
+    """Per-face edge-plane normals v_e x v_(e+1), computed once per mesh."""
+    hit = _FACE_PLANES.get(id(mesh))
+    if hit is None or hit[0] is not mesh:
+        triangles = mesh.vertices[mesh.faces]
+        hit = (mesh, tuple(np.ascontiguousarray(np.cross(triangles[:, e], triangles[:, (e + 1) % 3]))
+                           for e in range(3)))
+        _FACE_PLANES[id(mesh)] = hit
+        if len(_FACE_PLANES) > 16:
+            _FACE_PLANES.pop(next(iter(_FACE_PLANES)))
+    return hit[1]
+
+
 def face_index(mesh, latitude_deg: float, longitude_deg: float) -> int:
-    """Primal face containing a direction (reference sources.py:53-85)."""
+    """Primal face containing a direction (reference sources.py:53-85): among
+    the faces whose edge planes all hold the direction on one side (1e-12
+    slack), the one whose centre is nearest; the nearest centre if none does.
+
+    The planes are tested on the faces with the 24 nearest centres first: the
+    chosen face has the largest centre . direction of all passing faces, so
+    when any of those 24 passes, the choice is the one the full search makes;
+    otherwise every face is tested."""
     direction = geographic_direction(latitude_deg, longitude_deg)
-    triangles = mesh.vertices[mesh.faces]
-    signs = np.column_stack(tuple(
-        np.einsum("ij,j->i", np.cross(triangles[:, e], triangles[:, (e + 1) % 3]), direction)
-        for e in range(3)))
-    inside = np.all(signs >= -1.0e-12, axis=1) | np.all(signs <= 1.0e-12, axis=1)
-    candidates = np.flatnonzero(inside)
-    if len(candidates):
-        return int(candidates[np.argmax(mesh.face_centers[candidates] @ direction)])
-    return int(np.argmax(mesh.face_centers @ direction))
+    planes = _face_planes(mesh)
+    closeness = mesh.face_centers @ direction
+    subsets = [None]
+    if len(closeness) > 2 * _NEAREST_FACES:
+        subsets.insert(0, np.sort(np.argpartition(-closeness, _NEAREST_FACES)[:_NEAREST_FACES]))
+    for subset in subsets:
+        rows = subset if subset is not None else slice(None)
+        signs = np.column_stack(tuple(np.einsum("ij,j->i", p[rows], direction) for p in planes))
+        inside = np.all(signs >= -1.0e-12, axis=1) | np.all(signs <= 1.0e-12, axis=1)
+        candidates = np.flatnonzero(inside)
+        if len(candidates):
+            if subset is not None:
+                candidates = subset[candidates]
+            return int(candidates[np.argmax(mesh.face_centers[candidates] @ direction)])
+    return int(np.argmax(closeness))
 
 
 def geographic_face_index(simulation: "GeodesicFDTD", latitude_deg: float,

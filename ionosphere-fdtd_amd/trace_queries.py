@@ -13,15 +13,33 @@ nu = 1.8e11 exp(-0.15 z[km]), surface-impedance ground, a vertical Gaussian
 current at the source (default timing for its carrier), FP32, courant 0.35.
 The trace is E_r at the receiver after every 8th step of 4096 steps (512
 samples; step numbers, not seconds, since the time step follows the plasma
-CFL limit of each configuration).
+CFL limit of each configuration: 0.9 ns to 0.5 us across the box).
 
-In 4096 steps (0.02-0.8 ms) a signal covers at most ~240 km, so receivers
-drawn anywhere on the sphere would see nothing; queries therefore come from a
-station network -- eight transmitter sites, each with eight receivers within
-250 km.  A query is *in range* when its physical parameters lie in the
-ensemble box (h' 68-80 km, beta 0.25-0.55 /km, ground sigma 1e-4..1e-1 S/m,
-carrier 10-40 Hz) and its source/receiver pair is a station pair; the router
-answers other queries with FDTD.
+**Decomposition.**  The run is linear in its source, the source is injected
+at the three vertices of the mesh face containing it with barycentric weights
+s_j (sources.py staggered_distribution at the ground node), and the receiver
+reads the three vertices of its face with weights w_i.  So the FDTD trace of
+a query is exactly
+
+    trace = sum_j sum_i s_j w_i G(u_j, v_i),
+
+with G(u, v) the E_r trace at vertex v of the same configuration driven by a
+unit-weight source at vertex u.  The surrogate learns G for vertex pairs up to
+three mesh rings apart from the configuration and the pair's local geometry
+(distance, ring, dual-cell areas, edge lengths, degrees), and answers a query
+with the same weighted sum of predicted G.  Sources may sit anywhere on the
+sphere; a query is *in range* when its physics lies in the ensemble box
+(h' 68-80 km, beta 0.25-0.55 /km, ground sigma 1e-4..1e-1 S/m, carrier
+10-40 Hz), its receiver lies within 250 km of the source and every weighted
+(u, v) pair is within three rings (at s5 an edge is ~240 km; three rings hold
+every pair of every receiver within 300 km).  The router answers other
+queries with FDTD.
+
+**Router.**  A committee of networks, each trained on its own seeded half of
+the training runs, answers in-range queries with the mean of its members'
+traces; a query whose members disagree by more than `disagreement` (relative
+L2) goes to FDTD, and one configuration per batch is spot-checked by FDTD
+(any error above the tolerance sends the whole batch to FDTD).
 """
 
 from __future__ import annotations
@@ -49,51 +67,12 @@ STRIDE = 8
 SAMPLES = STEPS // STRIDE
 BOX = {"h_prime_km": (68.0, 80.0), "beta_per_km": (0.25, 0.55),
        "log10_ground_sigma": (-4.0, -1.0), "carrier_hz": (10.0, 40.0)}
-TRANSMITTERS = ((35.15, 126.85), (46.5, -90.9), (-33.9, 18.4), (64.8, -147.7),
-                (-12.0, -77.0), (28.6, 77.2), (51.5, -0.1), (-41.3, 174.8))
-STATION_OFFSETS = ((0.0, 0.0), (30.0, 45.0), (60.0, 90.0), (90.0, 135.0), (120.0, 180.0),
-                   (160.0, 225.0), (200.0, 270.0), (250.0, 315.0))  # (km, azimuth deg)
+MAX_DISTANCE_M = 250e3   # receivers in range: within this great-circle distance of the source
+RINGS = 3                # vertex pairs the surrogate covers
 COLUMNS = ("h_prime_km", "beta_per_km", "log10_ground_sigma", "src_lat", "src_lon", "rx_lat",
            "rx_lon", "carrier_hz")
 ELECTRON_CHARGE_C = 1.602_176_634e-19
 ELECTRON_MASS_KG = 9.109_383_713_9e-31
-
-
-def stations() -> np.ndarray:
-    """(transmitter, station, lat/lon) receiver positions."""
-    if _STATIONS is None:
-        out = np.zeros((len(TRANSMITTERS), len(STATION_OFFSETS), 2))
-        for t, (lat, lon) in enumerate(TRANSMITTERS):
-            for s, (d_km, az) in enumerate(STATION_OFFSETS):
-                out[t, s] = destination_point(lat, lon, az, d_km * 1e3, EARTH_RADIUS_M)
-        globals()["_STATIONS"] = out
-    return _STATIONS.copy()
-
-
-_STATIONS = None
-
-
-def sample_runs(n: int, seed: int) -> np.ndarray:
-    """(n, 5) runs: h', beta, log10 sigma_g, carrier, transmitter index."""
-    rng = np.random.default_rng(seed)
-    out = np.empty((n, 5))
-    out[:, 0] = rng.uniform(*BOX["h_prime_km"], n)
-    out[:, 1] = rng.uniform(*BOX["beta_per_km"], n)
-    out[:, 2] = rng.uniform(*BOX["log10_ground_sigma"], n)
-    out[:, 3] = rng.uniform(*BOX["carrier_hz"], n)
-    out[:, 4] = rng.integers(0, len(TRANSMITTERS), n)
-    return out
-
-
-def station_queries(runs: np.ndarray) -> np.ndarray:
-    """(n_runs * 8, 8) queries: every station of each run's transmitter."""
-    st = stations()
-    rows = []
-    for h, b, ls, f, t in runs:
-        tx = TRANSMITTERS[int(t)]
-        for s in range(len(STATION_OFFSETS)):
-            rows.append((h, b, ls, tx[0], tx[1], st[int(t), s, 0], st[int(t), s, 1], f))
-    return np.asarray(rows, dtype=np.float64)
 
 
 def _unit(lat, lon):
@@ -101,10 +80,136 @@ def _unit(lat, lon):
     return np.stack((np.cos(la) * np.cos(lo), np.cos(la) * np.sin(lo), np.sin(la)), axis=-1)
 
 
+def _latlon(unit):
+    u = np.asarray(unit, dtype=np.float64)
+    return (np.rad2deg(np.arcsin(np.clip(u[..., 2], -1.0, 1.0))),
+            np.rad2deg(np.arctan2(u[..., 1], u[..., 0])))
+
+
 def _altitudes():
     return np.linspace(0.0, 100e3, RADIAL_CELLS + 1)
 
 
+# --------------------------------------------------------------- sampling
+def _sample_physics(rng, n) -> np.ndarray:
+    out = np.empty((n, 4))
+    out[:, 0] = rng.uniform(*BOX["h_prime_km"], n)
+    out[:, 1] = rng.uniform(*BOX["beta_per_km"], n)
+    out[:, 2] = rng.uniform(*BOX["log10_ground_sigma"], n)
+    out[:, 3] = rng.uniform(*BOX["carrier_hz"], n)
+    return out
+
+
+def random_queries(n_configs: int, per_config: int, seed: int) -> np.ndarray:
+    """(n_configs * per_config, 8) queries: physics uniform in the box, the
+    source uniform on the sphere, each receiver uniform over the disc of
+    MAX_DISTANCE_M around its source (area-uniform distance, uniform azimuth)."""
+    rng = np.random.default_rng(seed)
+    phys = _sample_physics(rng, n_configs)
+    src_lat = np.rad2deg(np.arcsin(rng.uniform(-1.0, 1.0, n_configs)))
+    src_lon = rng.uniform(-180.0, 180.0, n_configs)
+    rows = []
+    for k in range(n_configs):
+        for _ in range(per_config):
+            d = MAX_DISTANCE_M * np.sqrt(rng.uniform())
+            az = rng.uniform(0.0, 360.0)
+            rlat, rlon = destination_point(src_lat[k], src_lon[k], az, d, EARTH_RADIUS_M)
+            h, b, ls, f = phys[k]
+            rows.append((h, b, ls, src_lat[k], src_lon[k], rlat, rlon, f))
+    return np.asarray(rows, dtype=np.float64)
+
+
+def training_runs(n: int, seed: int, mesh) -> np.ndarray:
+    """(n, 5) training runs: physics uniform in the box and a source vertex
+    drawn uniformly from the mesh vertices."""
+    rng = np.random.default_rng(seed)
+    out = np.empty((n, 5))
+    out[:, :4] = _sample_physics(rng, n)
+    out[:, 4] = rng.integers(0, mesh.n_vertices, n)
+    return out
+
+
+# ------------------------------------------------------- mesh neighbourhoods
+class MeshRings:
+    """Vertex rings (graph distance along primal edges) and the per-vertex
+    geometry the pair features use."""
+
+    def __init__(self, mesh, rings: int = RINGS):
+        self.mesh = mesh
+        self.rings = rings
+        V = mesh.n_vertices
+        e = np.asarray(mesh.edges, dtype=np.int64)
+        order = np.argsort(np.concatenate((e[:, 0], e[:, 1])), kind="stable")
+        ends = np.concatenate((e[:, 1], e[:, 0]))[order]
+        counts = np.bincount(np.concatenate((e[:, 0], e[:, 1])), minlength=V)
+        starts = np.concatenate(([0], np.cumsum(counts)))
+        self.adjacent = [ends[starts[v]:starts[v + 1]] for v in range(V)]
+        self.degree = counts
+        theta = np.asarray(mesh.primal_edge_angles, dtype=np.float64)
+        phi = np.asarray(mesh.dual_edge_angles, dtype=np.float64)
+        incident = np.concatenate((theta, theta))[order]
+        self.edge_length = np.add.reduceat(incident, starts[:-1]) / counts * EARTH_RADIUS_M
+        # phi / theta: the weight of an edge in the surface Laplacian (the
+        # coupling of its two vertices); per vertex, the mean over its edges
+        self.edge_coupling = phi / theta
+        incident_w = np.concatenate((self.edge_coupling, self.edge_coupling))[order]
+        self.coupling = np.add.reduceat(incident_w, starts[:-1]) / counts
+        self.mean_coupling = float(self.edge_coupling.mean())
+        self.edge_of = {(int(a), int(b)): k for k, (a, b) in enumerate(e)}
+        self.area = np.asarray(mesh.dual_cell_solid_angles, dtype=np.float64)
+        self.mean_area = float(self.area.mean())
+        self.mean_edge = float(self.edge_length.mean())
+        self.unit = np.asarray(mesh.vertices, dtype=np.float64)
+        self._hoods: dict[int, tuple[np.ndarray, np.ndarray]] = {}
+
+    def neighbourhood(self, u: int) -> tuple[np.ndarray, np.ndarray]:
+        """Vertices within `rings` rings of u (u first) and their ring numbers."""
+        hit = self._hoods.get(u)
+        if hit is not None:
+            return hit
+        ring = {u: 0}
+        frontier = [u]
+        for r in range(1, self.rings + 1):
+            nxt = []
+            for w in frontier:
+                for x in self.adjacent[w]:
+                    x = int(x)
+                    if x not in ring:
+                        ring[x] = r
+                        nxt.append(x)
+            frontier = nxt
+        verts = np.asarray(sorted(ring, key=lambda x: (ring[x], x)), dtype=np.int64)
+        out = (verts, np.asarray([ring[int(x)] for x in verts], dtype=np.int64))
+        self._hoods[u] = out
+        return out
+
+    def ring(self, u: int, v: int) -> int:
+        """Ring number of v around u, or rings + 1 beyond the neighbourhood."""
+        verts, rings = self.neighbourhood(int(u))
+        hit = np.flatnonzero(verts == int(v))
+        return int(rings[hit[0]]) if hit.size else self.rings + 1
+
+    def common_neighbours(self, u: int, v: int) -> int:
+        return len(np.intersect1d(self.adjacent[int(u)], self.adjacent[int(v)]))
+
+    def pair_coupling(self, u: int, v: int) -> float:
+        """phi / theta of the edge joining u and v (adjacent vertices)."""
+        k = self.edge_of.get((int(u), int(v)), self.edge_of.get((int(v), int(u))))
+        return float(self.edge_coupling[k])
+
+
+_RINGS: dict = {}
+
+
+def mesh_rings(mesh) -> MeshRings:
+    hit = _RINGS.get(id(mesh))
+    if hit is None or hit.mesh is not mesh:
+        hit = MeshRings(mesh)
+        _RINGS[id(mesh)] = hit
+    return hit
+
+
+# ---------------------------------------------------------------- physics
 def drude_profile(h_prime_km, beta_per_km, midpoint_altitudes_m):
     z = np.asarray(midpoint_altitudes_m) / 1000.0
     density = 1.43e13 * np.exp(-0.15 * h_prime_km) * np.exp((beta_per_km - 0.15) * (z - h_prime_km))
@@ -126,46 +231,43 @@ def time_step(h_prime_km, beta_per_km, mesh) -> float:
     return 0.35 * (min(geometric, 2.0 / wp) if wp > 0 else geometric)
 
 
-def encode(queries: np.ndarray, mesh) -> np.ndarray:
-    """(n, 8) queries -> (n, 16) FP32 features."""
-    q = np.asarray(queries, dtype=np.float64)
-    x = np.zeros((q.shape[0], 16))
-    x[:, 0] = (q[:, 0] - 74.0) / 6.0
-    x[:, 1] = (q[:, 1] - 0.40) / 0.15
-    x[:, 2] = (q[:, 2] + 2.5) / 1.5
-    x[:, 3] = (q[:, 7] - 25.0) / 15.0
-    src, rx = _unit(q[:, 3], q[:, 4]), _unit(q[:, 5], q[:, 6])
-    x[:, 4:7] = src
-    x[:, 7:10] = rx
-    x[:, 10:13] = (rx - src) * (EARTH_RADIUS_M / 100e3)
-    x[:, 13] = np.arccos(np.clip(np.sum(src * rx, axis=1), -1.0, 1.0)) * EARTH_RADIUS_M / 100e3
-    dts = {}
-    for i, (h, b) in enumerate(q[:, :2]):
-        key = (h, b)
-        if key not in dts:
-            dts[key] = time_step(h, b, mesh)
-        x[i, 14] = np.log(dts[key] / 1e-7)
+def pair_features(physics: np.ndarray, dts: np.ndarray, u: np.ndarray, v: np.ndarray,
+                  rings: MeshRings) -> np.ndarray:
+    """(n, 16) FP32 features of G(u, v): the configuration (normalised h',
+    beta, log sigma_g, carrier, log dt), the pair's great-circle distance and
+    ring, the local geometry of both vertices (dual-cell area, mean edge
+    length, mean Laplacian coupling phi/theta at u, pentagon count), the
+    coupling of the edge joining a ring-1 pair or the common neighbours of a
+    ring-2 pair, and the steps a wave needs to cover the distance."""
+    p = np.asarray(physics, dtype=np.float64)
+    u = np.asarray(u, dtype=np.int64)
+    v = np.asarray(v, dtype=np.int64)
+    x = np.zeros((p.shape[0], 16))
+    x[:, 0] = (p[:, 0] - 74.0) / 6.0
+    x[:, 1] = (p[:, 1] - 0.40) / 0.15
+    x[:, 2] = (p[:, 2] + 2.5) / 1.5
+    x[:, 3] = (p[:, 3] - 25.0) / 15.0
+    x[:, 4] = np.log(np.asarray(dts) / 1e-7)
+    dist = np.arccos(np.clip(np.sum(rings.unit[u] * rings.unit[v], axis=1), -1.0, 1.0)) \
+        * EARTH_RADIUS_M
+    ring = np.asarray([rings.ring(a, b) for a, b in zip(u, v)])
+    x[:, 5] = dist / 100e3
+    x[:, 6] = ring == 1
+    x[:, 7] = ring == 2
+    x[:, 8] = (rings.coupling[u] / rings.mean_coupling - 1.0) * 100.0
+    x[:, 9] = (rings.area[u] / rings.mean_area - 1.0) * 10.0
+    x[:, 10] = (rings.area[v] / rings.mean_area - 1.0) * 10.0
+    x[:, 11] = (rings.edge_length[u] / rings.mean_edge - 1.0) * 10.0
+    x[:, 12] = (rings.edge_length[v] / rings.mean_edge - 1.0) * 10.0
+    x[:, 13] = (rings.degree[u] == 5).astype(float) + (rings.degree[v] == 5)
+    x[:, 14] = [((rings.pair_coupling(a, b) / rings.mean_coupling - 1.0) * 10.0) if r == 1
+                else (rings.common_neighbours(a, b) / 2.0 if r == 2 else 0.0)
+                for a, b, r in zip(u, v, ring)]
+    x[:, 15] = np.log1p(dist / (C_0 * np.asarray(dts)) / STEPS)
     return x.astype(np.float32)
 
 
-def in_range(queries: np.ndarray, tolerance_deg: float = 1e-6) -> np.ndarray:
-    q = np.asarray(queries, dtype=np.float64)
-    ok = np.ones(q.shape[0], dtype=bool)
-    for j, key in enumerate(("h_prime_km", "beta_per_km", "log10_ground_sigma")):
-        lo, hi = BOX[key]
-        ok &= (q[:, j] >= lo) & (q[:, j] <= hi)
-    lo, hi = BOX["carrier_hz"]
-    ok &= (q[:, 7] >= lo) & (q[:, 7] <= hi)
-    st = stations()
-    pair = np.zeros(q.shape[0], dtype=bool)
-    for t, (lat, lon) in enumerate(TRANSMITTERS):
-        at_tx = (np.abs(q[:, 3] - lat) <= tolerance_deg) & (np.abs(q[:, 4] - lon) <= tolerance_deg)
-        for s in range(st.shape[1]):
-            pair |= at_tx & (np.abs(q[:, 5] - st[t, s, 0]) <= tolerance_deg) & \
-                (np.abs(q[:, 6] - st[t, s, 1]) <= tolerance_deg)
-    return ok & pair
-
-
+# ------------------------------------------------------------------ FDTD
 def build_run(mesh, h_prime_km, beta_per_km, log10_sigma, src_lat, src_lon, carrier_hz,
               dtype="float32") -> GeodesicFDTD:
     cfg = SimulationConfig(subdivision=SUBDIVISION, radial_cells=RADIAL_CELLS,
@@ -189,20 +291,131 @@ def build_run(mesh, h_prime_km, beta_per_km, log10_sigma, src_lat, src_lon, carr
 
 
 _RECEIVER_TABLES: dict = {}
+_FACE_TREES: dict = {}
+
+
+def _face_tree(mesh):
+    hit = _FACE_TREES.get(id(mesh))
+    if hit is None or hit[0] is not mesh:
+        from scipy.spatial import cKDTree
+
+        hit = (mesh, cKDTree(np.asarray(mesh.face_centers, dtype=np.float64)))
+        _FACE_TREES[id(mesh)] = hit
+    return hit[1]
+
+
+def surface_tables(mesh, lats, lons) -> list:
+    """Ground-level barycentric distributions (vertices, layer 0, weights) of
+    many points at once, equal to sources.barycentric_distribution point by
+    point: the containing face is searched among the 24 nearest face centres
+    (a k-d tree) with the same edge-plane test, and points without a passing
+    face there take the full per-point search."""
+    from .sources import _face_planes, face_index, geographic_direction
+
+    lats = np.asarray(lats, dtype=np.float64).ravel()
+    lons = np.asarray(lons, dtype=np.float64).ravel()
+    if lats.size == 0:
+        return []
+    dirs = np.stack([geographic_direction(a, b) for a, b in zip(lats, lons)])
+    k = min(24, mesh.n_faces)
+    _, near = _face_tree(mesh).query(dirs, k=k)
+    near = np.sort(np.asarray(near).reshape(len(dirs), k), axis=1)
+    planes = _face_planes(mesh)
+    signs = np.stack([np.einsum("nkj,nj->nk", p[near], dirs) for p in planes], axis=2)
+    inside = np.all(signs >= -1.0e-12, axis=2) | np.all(signs <= 1.0e-12, axis=2)
+    out = []
+    for i in range(len(dirs)):
+        cand = np.flatnonzero(inside[i])
+        if len(cand):
+            faces = near[i, cand]
+            face = int(faces[np.argmax(mesh.face_centers[faces] @ dirs[i])])
+        else:
+            face = face_index(mesh, lats[i], lons[i])
+        vertices = mesh.faces[face]
+        a, b, c = mesh.vertices[vertices]
+        normal = np.cross(b - a, c - a)
+        point = dirs[i] * float((normal @ a) / (normal @ dirs[i]))
+        ab, ac, off = b - a, c - a, point - a
+        d11, d12, d22 = float(ab @ ab), float(ab @ ac), float(ac @ ac)
+        o1, o2 = float(off @ ab), float(off @ ac)
+        den = d11 * d22 - d12 ** 2
+        wb = (d22 * o1 - d12 * o2) / den
+        wc = (d11 * o2 - d12 * o1) / den
+        weights = np.clip(np.asarray((1.0 - wb - wc, wb, wc)), 0.0, None)
+        out.append((vertices.copy(), 0, weights / weights.sum()))
+    return out
 
 
 def _receiver_table(mesh, altitudes, lat: float, lon: float):
-    """Surface receiver distribution at (lat, lon), kept per mesh: the station
-    network repeats the same few points in every query batch."""
+    """Surface receiver distribution at (lat, lon), kept per mesh."""
     key = (id(mesh), float(lat), float(lon))
     hit = _RECEIVER_TABLES.get(key)
     if hit is not None and hit[0] is mesh:
         return hit[1]
     table = barycentric_distribution(mesh, altitudes, lat, lon, 0.0)
-    _RECEIVER_TABLES[key] = (mesh, table)
-    if len(_RECEIVER_TABLES) > 4096:
-        _RECEIVER_TABLES.pop(next(iter(_RECEIVER_TABLES)))
+    _remember_table(mesh, key, table)
     return table
+
+
+def _remember_table(mesh, key, table) -> None:
+    _RECEIVER_TABLES[key] = (mesh, table)
+    if len(_RECEIVER_TABLES) > 16384:
+        _RECEIVER_TABLES.pop(next(iter(_RECEIVER_TABLES)))
+
+
+def _prefetch_tables(mesh, points) -> None:
+    """Fill the table cache for many (lat, lon) points with one batched search."""
+    missing = []
+    for lat, lon in points:
+        key = (id(mesh), float(lat), float(lon))
+        hit = _RECEIVER_TABLES.get(key)
+        if hit is None or hit[0] is not mesh:
+            missing.append(key)
+    missing = list(dict.fromkeys(missing))
+    if missing:
+        tables = surface_tables(mesh, [k[1] for k in missing], [k[2] for k in missing])
+        for key, table in zip(missing, tables):
+            _remember_table(mesh, key, table)
+
+
+def _run_ensembles(specs, mesh, batch, workers, progress=None):
+    """Advance runs in ensembles of `batch`; specs: (config key, receiver
+    table triple) per run, every table with the same receiver count.  The
+    next batch is built on host threads while the current one runs; the
+    ensembles then step by single launches, because a HIP graph capture on
+    this thread fails when other threads allocate device memory meanwhile.
+    Returns a list of (n_rx, SAMPLES) traces."""
+    import torch
+
+    # Initialise the device on this thread and let torch cache the device
+    # count: its uncached query (through amdsmi) reports no devices from the
+    # setup worker threads.
+    torch.cuda.init()
+    if torch.cuda.device_count() < 1:
+        raise RuntimeError("no ROCm device is visible")
+
+    def make(spec):
+        key, _ = spec
+        return build_run(mesh, key[0], key[1], key[2], key[3], key[4], key[5])
+
+    out = [None] * len(specs)
+    groups = [list(range(s, min(s + batch, len(specs)))) for s in range(0, len(specs), batch)]
+    with ThreadPoolExecutor(max_workers=max(1, workers)) as pool:
+        pending = pool.map(make, [specs[i] for i in groups[0]]) if groups else None
+        for gi, group in enumerate(groups):
+            built = list(pending)
+            if gi + 1 < len(groups):
+                pending = pool.map(make, [specs[i] for i in groups[gi + 1]])
+            overlapped = gi + 1 < len(groups)
+            ens = EnsembleFDTD(built, compile_chunk_size=1 if overlapped else 128,
+                               autotune=False)
+            traces = ens.record_er_observations([specs[i][1] for i in group], STEPS)
+            for k, i in enumerate(group):
+                out[i] = traces[k, STRIDE::STRIDE, :].T.astype(np.float64)
+            del ens, built
+            if progress is not None:
+                progress(group[-1] + 1, len(specs))
+    return out
 
 
 def fdtd_traces(queries: np.ndarray, mesh=None, *, batch: int = 64, workers: int = 8,
@@ -212,7 +425,7 @@ def fdtd_traces(queries: np.ndarray, mesh=None, *, batch: int = 64, workers: int
     (n, SAMPLES) FP64 traces (E_r after steps 8, 16, ..., 4096)."""
     mesh = mesh if mesh is not None else build_geodesic_mesh(SUBDIVISION)
     q = np.asarray(queries, dtype=np.float64)
-    keys = {}
+    keys: dict = {}
     order = []
     for i, row in enumerate(q):
         key = (row[0], row[1], row[2], row[3], row[4], row[7])
@@ -221,40 +434,110 @@ def fdtd_traces(queries: np.ndarray, mesh=None, *, batch: int = 64, workers: int
             order.append(key)
         keys[key].append(i)
     width = max(len(v) for v in keys.values())
-    out = np.zeros((q.shape[0], SAMPLES))
     alt = _altitudes()
-
-    def make(key):
-        sim = build_run(mesh, key[0], key[1], key[2], key[3], key[4], key[5])
+    _prefetch_tables(mesh, q[:, 5:7])
+    specs = []
+    for key in order:
         idx = keys[key]
         pts = [(q[i, 5], q[i, 6]) for i in idx]
         pts += [pts[0]] * (width - len(pts))  # equal receiver counts across the ensemble
         tables = [_receiver_table(mesh, alt, lat, lon) for lat, lon in pts]
-        rx = (np.asarray([t[0] for t in tables]), np.asarray([t[1] for t in tables]),
-              np.asarray([t[2] for t in tables]))
-        return sim, rx
-
-    import torch
-
-    # Initialise the device on this thread and let torch cache the device
-    # count: its uncached query (through amdsmi) reports no devices from the
-    # setup worker threads.
-    torch.cuda.init()
-    if torch.cuda.device_count() < 1:
-        raise RuntimeError("no ROCm device is visible")
-    with ThreadPoolExecutor(max_workers=max(1, min(workers, len(order)))) as pool:
-        for start in range(0, len(order), batch):
-            group = order[start:start + batch]
-            built = list(pool.map(make, group))
-            ens = EnsembleFDTD([b[0] for b in built], compile_chunk_size=128, autotune=False)
-            traces = ens.record_er_observations([b[1] for b in built], STEPS)
-            for k, key in enumerate(group):
-                for j, i in enumerate(keys[key]):
-                    out[i] = traces[k, STRIDE::STRIDE, j]
-            del ens, built
-            if progress is not None:
-                progress(min(start + batch, len(order)), len(order))
+        specs.append((key, (np.asarray([t[0] for t in tables]),
+                            np.asarray([t[1] for t in tables]),
+                            np.asarray([t[2] for t in tables]))))
+    traces = _run_ensembles(specs, mesh, batch, workers, progress)
+    out = np.zeros((q.shape[0], SAMPLES))
+    for key, tr in zip(order, traces):
+        for j, i in enumerate(keys[key]):
+            out[i] = tr[j]
     return out
+
+
+def pair_training_data(runs: np.ndarray, mesh, *, batch: int = 64, workers: int = 8,
+                       progress=None):
+    """G(u, v) for every vertex v within RINGS rings of each run's source
+    vertex u (the source placed exactly at u, receivers read exactly at v).
+    Returns (physics (P, 4), dt (P,), u (P,), v (P,), traces (P, SAMPLES))."""
+    rings = mesh_rings(mesh)
+    hoods = [rings.neighbourhood(int(r[4]))[0] for r in runs]
+    width = max(len(h) for h in hoods)
+    lat, lon = _latlon(rings.unit)
+    specs = []
+    for r, hood in zip(runs, hoods):
+        u = int(r[4])
+        verts = np.concatenate((hood, np.full(width - len(hood), u)))
+        table = (np.repeat(verts[:, None], 3, axis=1), np.zeros(width, dtype=np.int64),
+                 np.tile(np.array([1.0, 0.0, 0.0]), (width, 1)))
+        specs.append(((r[0], r[1], r[2], lat[u], lon[u], r[3]), table))
+    traces = _run_ensembles(specs, mesh, batch, workers, progress)
+    phys, dts, us, vs, ts = [], [], [], [], []
+    dt_cache: dict = {}
+    for r, hood, tr in zip(runs, hoods, traces):
+        key = (r[0], r[1])
+        if key not in dt_cache:
+            dt_cache[key] = time_step(r[0], r[1], mesh)
+        n = len(hood)
+        phys.append(np.repeat(r[None, :4], n, axis=0))
+        dts.append(np.full(n, dt_cache[key]))
+        us.append(np.full(n, int(r[4])))
+        vs.append(hood)
+        ts.append(tr[:n])
+    return (np.concatenate(phys), np.concatenate(dts), np.concatenate(us), np.concatenate(vs),
+            np.concatenate(ts))
+
+
+# ------------------------------------------------------- query decomposition
+def decompose(queries: np.ndarray, mesh):
+    """Split queries into weighted vertex pairs.
+
+    Returns (query index, u, v, weight s_j w_i) per pair with nonzero weight,
+    and an (n,) flag: the receiver lies within MAX_DISTANCE_M of the source and
+    every pair is within RINGS rings."""
+    q = np.asarray(queries, dtype=np.float64)
+    rings = mesh_rings(mesh)
+    alt = _altitudes()
+    qi, us, vs, ws = [], [], [], []
+    covered = np.ones(q.shape[0], dtype=bool)
+    src = _unit(q[:, 3], q[:, 4])
+    rx = _unit(q[:, 5], q[:, 6])
+    dist = np.arccos(np.clip(np.sum(src * rx, axis=1), -1.0, 1.0)) * EARTH_RADIUS_M
+    covered &= dist <= MAX_DISTANCE_M * (1.0 + 1e-9)
+    _prefetch_tables(mesh, np.concatenate((q[:, 3:5], q[:, 5:7])))
+    for i, row in enumerate(q):
+        su, _, sw = _receiver_table(mesh, alt, row[3], row[4])
+        rv, _, rw = _receiver_table(mesh, alt, row[5], row[6])
+        for u, a in zip(su, sw):
+            if a == 0.0:
+                continue
+            for v, b in zip(rv, rw):
+                if b == 0.0:
+                    continue
+                if rings.ring(int(u), int(v)) > rings.rings:
+                    covered[i] = False
+                qi.append(i)
+                us.append(int(u))
+                vs.append(int(v))
+                ws.append(a * b)
+    return (np.asarray(qi, dtype=np.int64), np.asarray(us, dtype=np.int64),
+            np.asarray(vs, dtype=np.int64), np.asarray(ws)), covered
+
+
+def in_range(queries: np.ndarray, mesh=None) -> np.ndarray:
+    """Physics in the ensemble box, receiver within MAX_DISTANCE_M of the
+    source, every weighted vertex pair within RINGS rings."""
+    mesh = mesh if mesh is not None else build_geodesic_mesh(SUBDIVISION)
+    q = np.asarray(queries, dtype=np.float64)
+    ok = np.ones(q.shape[0], dtype=bool)
+    for j, key in enumerate(("h_prime_km", "beta_per_km", "log10_ground_sigma")):
+        lo, hi = BOX[key]
+        ok &= (q[:, j] >= lo) & (q[:, j] <= hi)
+    lo, hi = BOX["carrier_hz"]
+    ok &= (q[:, 7] >= lo) & (q[:, 7] <= hi)
+    ok &= np.all(np.isfinite(q), axis=1) & (np.abs(q[:, [3, 5]]) <= 90.0).all(axis=1)
+    if ok.any():
+        _, covered = decompose(q[ok], mesh)
+        ok[np.flatnonzero(ok)] = covered
+    return ok
 
 
 def normalise(traces: np.ndarray, floor: float = 1e-30):
@@ -271,22 +554,58 @@ def trace_rel_l2(pred: np.ndarray, true: np.ndarray) -> np.ndarray:
     return num / np.maximum(den, 1e-300)
 
 
+# ---------------------------------------------------------------- networks
+def amplitude_basis(features: np.ndarray) -> np.ndarray:
+    """FP64 quadratic basis of the pair features (1, x_i, x_i x_j for i <= j):
+    the host-side baseline of the log amplitude."""
+    x = np.asarray(features, dtype=np.float64)
+    cols = [np.ones(x.shape[0])] + [x[:, i] for i in range(x.shape[1])]
+    for i in range(x.shape[1]):
+        for j in range(i, x.shape[1]):
+            cols.append(x[:, i] * x[:, j])
+    return np.stack(cols, axis=1)
+
+
+def fit_amplitude_baseline(features: np.ndarray, log_amp: np.ndarray,
+                           ridge: float = 1e-6) -> np.ndarray:
+    """Ridge least-squares coefficients of the log amplitude on amplitude_basis."""
+    b = amplitude_basis(features)
+    gram = b.T @ b
+    gram += ridge * b.shape[0] * np.eye(b.shape[1])
+    return np.linalg.solve(gram, b.T @ np.asarray(log_amp, dtype=np.float64))
+
+
 @dataclass
 class SurrogateModel:
-    """A trained network with its amplitude normalisation."""
+    """One trained network for G(u, v).
+
+    The pair traces' RMS amplitudes span ~17 decades across the box and the
+    rings.  The log amplitude is therefore split into an FP64 host baseline
+    (a quadratic least-squares fit on the pair features, `amp_coef`) and the
+    network's residual, normalised by `amp_mean` / `amp_std`: the network's
+    BF16 arithmetic then resolves a residual of ~0.1 natural-log units
+    instead of the whole range."""
 
     net: object
     amp_mean: float
     amp_std: float
+    amp_coef: np.ndarray | None = None
 
-    def predict(self, queries: np.ndarray, mesh) -> np.ndarray:
-        shape, amp_n = self.net.predict(encode(queries, mesh))
-        amp = amp_n.astype(np.float64) * self.amp_std + self.amp_mean
+    def log_amplitude(self, features: np.ndarray, amp_n: np.ndarray) -> np.ndarray:
+        amp = np.asarray(amp_n, dtype=np.float64) * self.amp_std + self.amp_mean
+        if self.amp_coef is not None:
+            amp = amp + amplitude_basis(features) @ self.amp_coef
+        return amp
+
+    def predict_pairs(self, features: np.ndarray) -> np.ndarray:
+        shape, amp_n = self.net.predict(features)
+        amp = self.log_amplitude(features, amp_n)
         return shape.astype(np.float64) * np.exp(amp)[:, None]
 
     def save(self, path) -> None:
         params = self.net.host_parameters()
-        np.savez(path, amp_mean=self.amp_mean, amp_std=self.amp_std,
+        extra = {} if self.amp_coef is None else {"amp_coef": self.amp_coef}
+        np.savez(path, amp_mean=self.amp_mean, amp_std=self.amp_std, **extra,
                  **{f"param_{k}": v for k, v in params.items()})
 
     @classmethod
@@ -296,17 +615,54 @@ class SurrogateModel:
         with np.load(path, allow_pickle=False) as data:
             params = {k[len("param_"):]: data[k] for k in data.files if k.startswith("param_")}
             mean, std = float(data["amp_mean"]), float(data["amp_std"])
-        return cls(TraceSurrogate(SurrogateConfig(), params), mean, std)
+            coef = np.asarray(data["amp_coef"]) if "amp_coef" in data.files else None
+        return cls(TraceSurrogate(SurrogateConfig(), params), mean, std, coef)
+
+
+class SurrogateCommittee:
+    """Independently trained networks answering queries through the vertex-pair
+    decomposition; their disagreement is the router's uncertainty."""
+
+    def __init__(self, members: list[SurrogateModel]):
+        if not members:
+            raise ValueError("a committee needs at least one member")
+        self.members = list(members)
+
+    @classmethod
+    def load(cls, paths):
+        return cls([SurrogateModel.load(p) for p in paths])
+
+    def predict(self, queries: np.ndarray, mesh) -> tuple[np.ndarray, np.ndarray]:
+        """(mean trace (n, SAMPLES), member disagreement (n,)) for in-range queries:
+        the disagreement is the largest relative L2 distance of a member's
+        trace from the committee mean."""
+        q = np.asarray(queries, dtype=np.float64)
+        (qi, u, v, w), _ = decompose(q, mesh)
+        dts = np.asarray([time_step(h, b, mesh) for h, b in q[:, :2]])
+        feats = pair_features(q[qi][:, [0, 1, 2, 7]], dts[qi], u, v, mesh_rings(mesh))
+        per_member = []
+        for m in self.members:
+            g = m.predict_pairs(feats) * w[:, None]
+            t = np.zeros((q.shape[0], SAMPLES))
+            np.add.at(t, qi, g)
+            per_member.append(t)
+        mean = np.mean(per_member, axis=0)
+        spread = np.max([trace_rel_l2(t, mean) for t in per_member], axis=0)
+        return mean, spread
 
 
 def train(features: np.ndarray, traces: np.ndarray, *, steps: int = 6000, batch: int = 256,
           lr: float = 2e-3, seed: int = 0, log=None, validation=None) -> SurrogateModel:
-    """Fit the surrogate to (features, traces) with Adam and a cosine schedule."""
+    """Fit one network to (features, traces) with Adam and a cosine schedule.
+    `validation` is (queries, FDTD traces, mesh): query-level errors of the
+    network through the pair decomposition are logged during training."""
     import torch
 
     from .surrogate import TraceSurrogate
 
     shape, amp = normalise(traces)
+    coef = fit_amplitude_baseline(features, amp)
+    amp = amp - amplitude_basis(features) @ coef
     mean, std = float(amp.mean()), float(amp.std() + 1e-12)
     net = TraceSurrogate(seed=seed)
     dev = net.device
@@ -320,31 +676,38 @@ def train(features: np.ndarray, traces: np.ndarray, *, steps: int = 6000, batch:
         rate = lr * 0.5 * (1.0 + np.cos(np.pi * step / steps)) + 1e-6
         loss = net.train_step(x.index_select(0, idx), s.index_select(0, idx),
                               a.index_select(0, idx), rate)
-        if log is not None and (step % 500 == 0 or step == steps - 1):
+        if log is not None and (step % 1000 == 0 or step == steps - 1):
             msg = ""
-            if validation is not None and (step % 2000 == 0 or step == steps - 1):
+            if validation is not None and (step % 5000 == 0 or step == steps - 1):
                 vq, vt, vmesh = validation
-                err = trace_rel_l2(SurrogateModel(net, mean, std).predict(vq, vmesh), vt)
+                pred, _ = SurrogateCommittee([SurrogateModel(net, mean, std, coef)]).predict(
+                    vq, vmesh)
+                err = trace_rel_l2(pred, vt)
                 msg = f" validation median {np.median(err):.4f} p90 {np.quantile(err, 0.9):.4f}"
             log(step, loss, msg)
-    return SurrogateModel(net, mean, std)
+    return SurrogateModel(net, mean, std, coef)
 
 
 class TraceQueryService:
-    """Answer trace queries with the surrogate, FDTD staying the judge.
+    """Answer trace queries with the surrogate committee, FDTD staying the judge.
 
-    In-range queries get the surrogate's trace unless the batch's spot check
-    fails: one configuration of the batch, chosen at random, is simulated and
-    every in-range query of that configuration is compared with FDTD; any
-    error above `tolerance` sends the whole batch to FDTD.  Out-of-range
-    queries always go to FDTD.
+    In-range queries get the committee's mean trace unless (a) its members
+    disagree by more than `disagreement` -- that query goes to FDTD -- or (b)
+    the batch's spot check fails: `spot_checks` configurations of the batch,
+    chosen at random among the confident queries, are simulated and every
+    confident query of those configurations is compared with FDTD; any error
+    above `tolerance` sends the whole batch to FDTD.  Out-of-range queries
+    always go to FDTD.
     """
 
-    def __init__(self, model: SurrogateModel, mesh=None, *, tolerance: float = 0.02,
-                 spot_checks: int = 1, seed: int = 0):
+    def __init__(self, model: SurrogateCommittee, mesh=None, *, tolerance: float = 0.02,
+                 disagreement: float = 0.01, spot_checks: int = 1, seed: int = 0):
+        if isinstance(model, SurrogateModel):
+            model = SurrogateCommittee([model])
         self.model = model
         self.mesh = mesh if mesh is not None else build_geodesic_mesh(SUBDIVISION)
         self.tolerance = tolerance
+        self.disagreement = disagreement
         self.spot_checks = spot_checks
         self.rng = np.random.default_rng(seed)
 
@@ -353,79 +716,103 @@ class TraceQueryService:
         t0 = time.perf_counter()
         out = np.zeros((q.shape[0], SAMPLES))
         served = np.full(q.shape[0], "fdtd", dtype=object)
-        ok = in_range(q)
+        ok = in_range(q, self.mesh)
         report = {"queries": int(q.shape[0]), "in_range": int(ok.sum())}
         fdtd_rows = np.flatnonzero(~ok)
         if ok.any():
             rows = np.flatnonzero(ok)
             t1 = time.perf_counter()
-            pred = self.model.predict(q[rows], self.mesh)
+            pred, spread = self.model.predict(q[rows], self.mesh)
             report["surrogate_seconds"] = time.perf_counter() - t1
-            keys = [tuple(q[i, [0, 1, 2, 3, 4, 7]]) for i in rows]
-            distinct = sorted(set(keys))
-            picks = self.rng.choice(len(distinct), size=min(self.spot_checks, len(distinct)),
-                                    replace=False)
-            chosen = {distinct[p] for p in picks}
-            check = np.asarray([k in chosen for k in keys])
-            t2 = time.perf_counter()
-            truth = fdtd_traces(q[rows[check]], self.mesh, batch=64)
-            report["spot_check_seconds"] = time.perf_counter() - t2
-            errors = trace_rel_l2(pred[check], truth)
-            report["spot_check_errors"] = errors.tolist()
-            if np.all(errors <= self.tolerance):
-                out[rows] = pred
-                out[rows[check]] = truth
-                served[rows] = "surrogate"
-                served[rows[check]] = "fdtd"
-            else:
-                fdtd_rows = np.concatenate((fdtd_rows, rows))
-                report["fallback"] = "spot check failed"
+            confident = spread <= self.disagreement
+            report["uncertain"] = int(np.sum(~confident))
+            fdtd_rows = np.concatenate((fdtd_rows, rows[~confident]))
+            rows, pred = rows[confident], pred[confident]
+            if rows.size:
+                keys = [tuple(q[i, [0, 1, 2, 3, 4, 7]]) for i in rows]
+                distinct = sorted(set(keys))
+                picks = self.rng.choice(len(distinct), size=min(self.spot_checks, len(distinct)),
+                                        replace=False)
+                chosen = {distinct[p] for p in picks}
+                check = np.asarray([k in chosen for k in keys])
+                t2 = time.perf_counter()
+                truth = fdtd_traces(q[rows[check]], self.mesh, batch=64)
+                report["spot_check_seconds"] = time.perf_counter() - t2
+                errors = trace_rel_l2(pred[check], truth)
+                report["spot_check_errors"] = errors.tolist()
+                if np.all(errors <= self.tolerance):
+                    out[rows] = pred
+                    out[rows[check]] = truth
+                    served[rows] = "surrogate"
+                    served[rows[check]] = "fdtd"
+                else:
+                    fdtd_rows = np.concatenate((fdtd_rows, rows))
+                    report["fallback"] = "spot check failed"
         if fdtd_rows.size:
+            t3 = time.perf_counter()
             out[fdtd_rows] = fdtd_traces(q[fdtd_rows], self.mesh)
+            report["fallback_fdtd_seconds"] = time.perf_counter() - t3
         report["seconds"] = time.perf_counter() - t0
         report["served_by_surrogate"] = int(np.sum(served == "surrogate"))
         return out, served, report
 
 
 def _main(argv=None) -> int:
-    """Generate C2-style training data on the GPU, train, save the model."""
+    """Generate pair training data on the GPU, train one committee member, save it."""
     import argparse
     import json
     import sys
 
     ap = argparse.ArgumentParser(prog="python -m ionosphere_fdtd_amd.trace_queries")
-    ap.add_argument("--runs", type=int, default=8192)
-    ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--runs", type=int, default=10000)
+    ap.add_argument("--seed", type=int, default=20261205)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", required=True)
-    ap.add_argument("--validation-runs", type=int, default=128)
+    ap.add_argument("--validation-configs", type=int, default=64)
+    ap.add_argument("--max-seconds", type=float, default=None,
+                    help="stop generating runs once this much time has passed (the runs "
+                         "generated so far are used)")
     args = ap.parse_args(argv)
     mesh = build_geodesic_mesh(SUBDIVISION)
     t0 = time.perf_counter()
-    runs = sample_runs(args.runs + args.validation_runs, args.seed)
-    queries = station_queries(runs)
-
-    def progress(done, total):
-        print(f"fdtd {done}/{total} runs, {time.perf_counter() - t0:.0f} s", file=sys.stderr,
-              flush=True)
-
-    traces = fdtd_traces(queries, mesh, progress=progress)
+    runs = training_runs(args.runs, args.seed, mesh)
+    if args.max_seconds is not None:
+        # time a first slice to size the rest of the generation
+        probe = pair_training_data(runs[:256], mesh)
+        rate = 256 / (time.perf_counter() - t0)
+        keep = max(256, min(args.runs, int(rate * args.max_seconds)))
+        print(f"{rate:.1f} runs/s: generating {keep} runs", file=sys.stderr, flush=True)
+        rest = pair_training_data(runs[256:keep], mesh, progress=lambda d, n: (
+            d % 1024 == 0 and print(f"fdtd {256 + d}/{keep} runs, "
+                                    f"{time.perf_counter() - t0:.0f} s", file=sys.stderr,
+                                    flush=True))) if keep > 256 else None
+        data = probe if rest is None else tuple(np.concatenate((a, b)) for a, b in zip(probe, rest))
+        runs = runs[:keep]
+    else:
+        data = pair_training_data(runs, mesh, progress=lambda d, n: (
+            d % 1024 == 0 and print(f"fdtd {d}/{n} runs, {time.perf_counter() - t0:.0f} s",
+                                    file=sys.stderr, flush=True)))
+    phys, dts, us, vs, traces = data
     gen = time.perf_counter() - t0
-    feats = encode(queries, mesh)
-    n_train = args.runs * len(STATION_OFFSETS)
+    feats = pair_features(phys, dts, us, vs, mesh_rings(mesh))
+    vq = random_queries(args.validation_configs, 8, seed=args.seed + 1)
+    vt = fdtd_traces(vq, mesh)
     t1 = time.perf_counter()
-    model = train(feats[:n_train], traces[:n_train], steps=args.steps, batch=args.batch,
-                  lr=args.lr, seed=1, validation=(queries[n_train:], traces[n_train:], mesh),
+    model = train(feats, traces, steps=args.steps, batch=args.batch, lr=args.lr,
+                  seed=args.seed % 1000, validation=(vq, vt, mesh),
                   log=lambda s, l, m: print(f"train step {s} loss {l:.5f}{m}", file=sys.stderr,
                                             flush=True))
     fit = time.perf_counter() - t1
     model.save(args.out)
-    err = trace_rel_l2(model.predict(queries[n_train:], mesh), traces[n_train:])
-    print(json.dumps({"runs": args.runs, "traces": n_train, "generate_s": round(gen, 1),
-                      "train_s": round(fit, 1), "validation_median": float(np.median(err)),
+    pred, _ = SurrogateCommittee([model]).predict(vq, mesh)
+    err = trace_rel_l2(pred, vt)
+    print(json.dumps({"runs": int(len(runs)), "pairs": int(len(traces)),
+                      "generate_s": round(gen, 1), "train_s": round(fit, 1),
+                      "validation_median": float(np.median(err)),
                       "validation_p90": float(np.quantile(err, 0.9)),
+                      "validation_p99": float(np.quantile(err, 0.99)),
                       "validation_within_2pct": float(np.mean(err <= 0.02))}))
     return 0
 

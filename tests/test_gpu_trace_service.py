@@ -1,10 +1,18 @@
-"""The trained trace surrogate and the query router on held-out queries.
+"""Trace queries: the vertex-pair decomposition, the trained surrogate
+committee and the query router, on held-out random-geometry queries.
 
-Bars: on 16 held-out configurations (128 station queries, seeds disjoint
-from training) the trained network is within 2 % relative L2 of FDTD for the
-median query and within 5 % at the 90th percentile, while the same network
-untrained is off by more than 50 % -- what the training buys; out-of-range
-queries and batches whose spot check fails are answered by FDTD exactly.
+Bars:
+  * decomposition: the FDTD trace of a query equals the weighted sum of the
+    FDTD pair responses G(u_j, v_i) to FP32 roundoff (1e-5 relative L2): the
+    run is linear in its source and both ends are barycentric combinations
+    of vertices;
+  * on 16 held-out configurations (sources anywhere on the sphere, 8
+    receivers each within 250 km; seeds disjoint from training) the trained
+    committee is within 2 % relative L2 of FDTD for the median query and
+    within 5 % at the 90th percentile, while an untrained network is off by
+    more than 50 % -- what the training buys;
+  * out-of-range queries, queries the committee disagrees on and batches
+    whose spot check fails are answered by FDTD exactly.
 """
 
 from __future__ import annotations
@@ -16,7 +24,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-MODEL = Path(__file__).resolve().parents[1] / "ionosphere-fdtd_amd" / "data" / "trace_surrogate.npz"
+DATA = Path(__file__).resolve().parents[1] / "ionosphere-fdtd_amd" / "data"
+MODELS = sorted(DATA.glob("trace_surrogate_*.npz"))
 
 
 @pytest.fixture(scope="module")
@@ -25,20 +34,48 @@ def held_out():
     from ionosphere_fdtd_amd import trace_queries as tq
 
     mesh = g.build_geodesic_mesh(tq.SUBDIVISION)
-    queries = tq.station_queries(tq.sample_runs(16, seed=777))
+    queries = tq.random_queries(16, 8, seed=777)
     return mesh, queries, tq.fdtd_traces(queries, mesh)
 
 
-def test_trained_surrogate_matches_fdtd_and_beats_the_untrained_network(held_out):
+def test_pair_decomposition_reproduces_fdtd(held_out):
+    from ionosphere_fdtd_amd import trace_queries as tq
+
+    mesh, queries, truth = held_out
+    q = queries[::8][:4]  # one receiver of four configurations
+    want = truth[::8][:4]
+    (qi, u, v, w), covered = tq.decompose(q, mesh)
+    assert covered.all()
+    # one pair run per (configuration, source vertex), every needed receiver vertex
+    runs = np.asarray([(*q[i, [0, 1, 2, 7]], uu) for i, uu in sorted(set(zip(qi, u)))])
+    phys, dts, us, vs, traces = tq.pair_training_data(runs, mesh, batch=16)
+    lookup = {}
+    for p, a, b, t in zip(phys, us, vs, traces):
+        lookup[(tuple(p), int(a), int(b))] = t
+    got = np.zeros_like(want)
+    for i, a, b, wt in zip(qi, u, v, w):
+        got[i] += wt * lookup[(tuple(q[i, [0, 1, 2, 7]]), int(a), int(b))]
+    err = tq.trace_rel_l2(got, want)
+    print("decomposition rel-L2", err, flush=True)
+    assert np.all(err <= 1e-5)
+
+
+def test_trained_committee_matches_fdtd_and_beats_an_untrained_network(held_out):
     from ionosphere_fdtd_amd import trace_queries as tq
     from ionosphere_fdtd_amd.surrogate import TraceSurrogate
 
     mesh, queries, truth = held_out
-    model = tq.SurrogateModel.load(MODEL)
-    err = tq.trace_rel_l2(model.predict(queries, mesh), truth)
-    untrained = tq.SurrogateModel(TraceSurrogate(seed=3), model.amp_mean, model.amp_std)
-    err0 = tq.trace_rel_l2(untrained.predict(queries, mesh), truth)
-    print(f"trained median {np.median(err):.4f} p90 {np.quantile(err, 0.9):.4f}; "
+    assert len(MODELS) >= 2
+    committee = tq.SurrogateCommittee.load(MODELS)
+    pred, spread = committee.predict(queries, mesh)
+    err = tq.trace_rel_l2(pred, truth)
+    m0 = committee.members[0]
+    # the same host amplitude baseline with an untrained network
+    untrained = tq.SurrogateCommittee([tq.SurrogateModel(TraceSurrogate(seed=3), m0.amp_mean,
+                                                         m0.amp_std, m0.amp_coef)])
+    err0 = tq.trace_rel_l2(untrained.predict(queries, mesh)[0], truth)
+    print(f"trained median {np.median(err):.4f} p90 {np.quantile(err, 0.9):.4f} "
+          f"max {err.max():.4f} spread median {np.median(spread):.4f}; "
           f"untrained median {np.median(err0):.3f}", flush=True)
     assert np.median(err) <= 0.02 and np.quantile(err, 0.9) <= 0.05
     assert np.median(err0) >= 0.5
@@ -49,25 +86,38 @@ def test_router_serves_in_range_queries_and_falls_back_to_fdtd(held_out):
     from ionosphere_fdtd_amd.surrogate import TraceSurrogate
 
     mesh, queries, truth = held_out
-    model = tq.SurrogateModel.load(MODEL)
-    service = tq.TraceQueryService(model, mesh, tolerance=0.05, spot_checks=1, seed=1)
+    committee = tq.SurrogateCommittee.load(MODELS)
+    service = tq.TraceQueryService(committee, mesh, tolerance=0.05, disagreement=1.0,
+                                   spot_checks=1, seed=1)
     answers, served, report = service.answer(queries)
     print(report, flush=True)
     assert report["in_range"] == len(queries)
     assert np.sum(served == "surrogate") == len(queries) - 8  # one spot-checked configuration
     checked = served == "fdtd"
     np.testing.assert_array_equal(answers[checked], truth[checked])
-    # out-of-range physics and an off-network receiver go to FDTD
+    # queries the committee disagrees on go to FDTD
+    strict = tq.TraceQueryService(committee, mesh, tolerance=0.05, disagreement=0.0,
+                                  spot_checks=1, seed=1)
+    s_answers, s_served, s_report = strict.answer(queries[:16])
+    assert s_report["uncertain"] == 16 and np.all(s_served == "fdtd")
+    np.testing.assert_array_equal(s_answers, truth[:16])
+    # out-of-range physics and a receiver beyond 250 km go to FDTD
+    from ionosphere_fdtd_amd.sources import destination_point
+
     odd = queries[:4].copy()
     odd[:2, 1] = 0.7
-    odd[2:, 5] += 0.5
-    assert not tq.in_range(odd).any()
+    for k in (2, 3):
+        odd[k, 5], odd[k, 6] = destination_point(odd[k, 3], odd[k, 4], 30.0, 400e3,
+                                                 tq.EARTH_RADIUS_M)
+    assert not tq.in_range(odd, mesh).any()
     o_answers, o_served, _ = service.answer(odd)
     assert np.all(o_served == "fdtd")
     np.testing.assert_array_equal(o_answers, tq.fdtd_traces(odd, mesh))
     # an untrained network fails its spot check: the whole batch falls back
-    bad = tq.TraceQueryService(tq.SurrogateModel(TraceSurrogate(seed=3), model.amp_mean,
-                                                 model.amp_std), mesh, tolerance=0.02)
+    m0 = committee.members[0]
+    bad = tq.TraceQueryService(tq.SurrogateModel(TraceSurrogate(seed=3), m0.amp_mean,
+                                                 m0.amp_std, m0.amp_coef), mesh, tolerance=0.02,
+                               disagreement=1.0)
     b_answers, b_served, b_report = bad.answer(queries[:16])
     assert b_report.get("fallback") == "spot check failed"
     assert np.all(b_served == "fdtd")

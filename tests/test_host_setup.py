@@ -319,3 +319,33 @@ def test_nearest_narrow_storage_with_slow_source_warns(monkeypatch):
     with pytest.raises(ValueError, match="storage_rounding"):
         g.GeodesicFDTD(g.SimulationConfig(subdivision=0, radial_cells=2), dtype="float16",
                        storage_rounding="up")
+
+
+def test_trace_query_rings_and_pair_weights():
+    """Vertex rings equal breadth-first graph distances along primal edges,
+    and a query's vertex-pair weights s_j w_i sum to one."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import shortest_path
+
+    from ionosphere_fdtd_amd import trace_queries as tq
+
+    mesh = g.build_geodesic_mesh(3)
+    rings = tq.MeshRings(mesh)
+    e = np.asarray(mesh.edges)
+    adj = coo_matrix((np.ones(len(e)), (e[:, 0], e[:, 1])), shape=(mesh.n_vertices,) * 2)
+    dist = shortest_path(adj, directed=False, unweighted=True)
+    for u in (0, 11, 137, mesh.n_vertices - 1):
+        verts, ring = rings.neighbourhood(u)
+        assert np.array_equal(np.sort(verts), np.flatnonzero(dist[u] <= tq.RINGS))
+        assert np.array_equal(ring, dist[u, verts].astype(int))
+        assert verts[0] == u and np.all(np.diff(ring) >= 0)
+    s5 = g.build_geodesic_mesh(tq.SUBDIVISION)
+    q = tq.random_queries(6, 3, seed=5)
+    (qi, u, v, w), covered = tq.decompose(q, s5)
+    assert covered.all()
+    np.testing.assert_allclose(np.bincount(qi, weights=w, minlength=len(q)), 1.0, atol=1e-12)
+    far = q[:1].copy()
+    far[0, 5], far[0, 6] = tq.destination_point(far[0, 3], far[0, 4], 0.0, 300e3,
+                                                tq.EARTH_RADIUS_M)
+    assert not tq.in_range(far, s5)[0]
+    assert tq.in_range(q, s5).all()
