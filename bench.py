@@ -436,7 +436,7 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=2000, help="held-out queries (--workload c4)")
     ap.add_argument("--receivers-per-config", type=int, default=1,
                     help="queries sharing one configuration (--workload c4)")
-    ap.add_argument("--disagreement", default="0.01",
+    ap.add_argument("--disagreement", default="0.015,0.01",
                     help="committee disagreement above which a query goes to FDTD "
                          "(--workload c4); a comma-separated list times the service at "
                          "each, the first one reported as the result")
