@@ -252,6 +252,25 @@ int geo_plan_set_step_index(GeoPlan* plan, int64_t step_index, void* stream);
  * weights. */
 int geo_set_receivers(GeoPlan* plan, const int64_t* vertices, const int64_t* layers,
                       const double* weights, int32_t n_receivers);
+/* H receivers for geo_step_h: (n_face_receivers, face_width) faces, radial
+ * midpoint layers and weights, and (n_edge_receivers, edge_width) edges,
+ * radial node layers and weights.  Replaces the argument validation and
+ * upload of record_h_observations (solver.py:1158-1203). */
+int geo_set_h_receivers(GeoPlan* plan, const int64_t* faces, const int64_t* face_layers,
+                        const double* face_weights, int32_t n_face_receivers, int32_t face_width,
+                        const int64_t* edges, const int64_t* edge_layers,
+                        const double* edge_weights, int32_t n_edge_receivers, int32_t edge_width);
+/* Advance n_steps like geo_step while writing H traces
+ * (record_h_observations, solver.py:1138-1229): hr_trace (rows, n_face_receivers)
+ * and ht_trace (rows, n_edge_receivers), compute type, either may be NULL,
+ * with rows = n_steps / sample_every + 1 (+1 when sample_every does not
+ * divide n_steps).  Row 0 is the H state before the first step, row k the
+ * state after step k * sample_every, the last row the state after the last
+ * step: H**(k-1/2) in the reference's half-step labelling.  Each sample is
+ * sum_q w[i,q] * H[entity[i,q], layer[i,q]], written by one block of the E
+ * launch (which does not write H), so graph chunks replay unchanged. */
+int geo_step_h(GeoPlan* plan, const void* currents, int64_t n_steps, int32_t chunk,
+               int32_t sample_every, void* hr_trace, void* ht_trace, void* stream);
 /* Write one trace row (n_receivers compute-type values) from the current Er. */
 int geo_sample_er(GeoPlan* plan, void* row, void* stream);
 /* FP64 reductions over the bound state; synchronises the stream. */

@@ -110,7 +110,7 @@ class GeoProfiles(C.Structure):
 
 EXPORTED_SYMBOLS = (
     "geo_plan_create", "geo_plan_layout", "geo_plan_bind", "geo_step", "geo_step_prepare",
-    "geo_plan_set_step_index", "geo_set_receivers",
+    "geo_plan_set_step_index", "geo_set_receivers", "geo_set_h_receivers", "geo_step_h",
     "geo_sample_er", "geo_diagnostics", "geo_radial_profiles", "geo_time_kernels",
     "geo_plan_destroy",
     "geo_last_error", "geo_version", "geo_stream_probe", "geo_get_storage_scales",
@@ -145,6 +145,9 @@ def load_library(path: str | os.PathLike | None = None) -> C.CDLL:
     lib.geo_step_prepare.argtypes = [vp, C.c_int32]
     lib.geo_plan_set_step_index.argtypes = [vp, C.c_int64, vp]
     lib.geo_set_receivers.argtypes = [vp, _p_i64, _p_i64, _p_f64, C.c_int32]
+    lib.geo_set_h_receivers.argtypes = [vp, _p_i64, _p_i64, _p_f64, C.c_int32, C.c_int32,
+                                        _p_i64, _p_i64, _p_f64, C.c_int32, C.c_int32]
+    lib.geo_step_h.argtypes = [vp, vp, C.c_int64, C.c_int32, C.c_int32, vp, vp, vp]
     lib.geo_sample_er.argtypes = [vp, vp, vp]
     lib.geo_diagnostics.argtypes = [vp, C.POINTER(GeoDiagOut), vp]
     lib.geo_radial_profiles.argtypes = [vp, C.POINTER(GeoProfiles), vp]

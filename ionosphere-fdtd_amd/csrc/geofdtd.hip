@@ -150,6 +150,11 @@ struct PlanBase {
   virtual int prepare(int32_t chunk) = 0;
   virtual int set_step_index(int64_t index, hipStream_t stream) = 0;
   virtual int set_receivers(const int64_t* v, const int64_t* l, const double* w, int32_t n) = 0;
+  virtual int set_h_receivers(const int64_t* f, const int64_t* fl, const double* fw, int32_t nf,
+                              int32_t wf, const int64_t* e, const int64_t* el, const double* ew,
+                              int32_t ne, int32_t we) = 0;
+  virtual int step_h(const void* currents, int64_t n, int32_t chunk, int32_t every, void* rt,
+                     void* tt, hipStream_t stream) = 0;
   virtual int sample(void* row, hipStream_t stream) = 0;
   virtual int diagnostics(GeoDiagOut* out, hipStream_t stream) = 0;
   virtual int radial_profiles(const GeoProfiles* out, hipStream_t stream) = 0;
@@ -532,6 +537,7 @@ struct Plan final : PlanBase {
   // receivers (owned separately so repeated record calls do not grow the arena)
   int32_t n_rx = 0;
   void* rx_mem = nullptr;
+  void* hrx_mem = nullptr;  // H receiver tables
   // permittivity for energy diagnostics (absolute, F/m), rows 1 or entity count
   double* eps_er_dev = nullptr;
   double* eps_et_dev = nullptr;
@@ -561,6 +567,7 @@ struct Plan final : PlanBase {
 
   ~Plan() override {
     if (rx_mem) (void)hipFree(rx_mem);
+    if (hrx_mem) (void)hipFree(hrx_mem);
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -1024,6 +1031,7 @@ struct Plan final : PlanBase {
     {
       std::vector<SegSpec> segs = {{geo::kSegEVertex, geom_blocks(ep.g_vertex)},
                                    {geo::kSegEEdge, geom_blocks(ep.g_edge)}};
+      segs.push_back({geo::kSegHSample, 1});  // H traces; returns at once without them
       std::vector<uint32_t> map = build_block_map(segs);
       uint32_t* d_map;
       if ((rc = arena.upload(map, &d_map)) != GEO_OK) return rc;
@@ -1179,6 +1187,58 @@ struct Plan final : PlanBase {
     return GEO_OK;
   }
 
+  // H receivers (record_h_observations, solver.py:1138-1229): rows of
+  // `width` (entity, layer, weight) triples, faces on midpoint layers and
+  // edges on node layers.
+  int set_h_receivers(const int64_t* f, const int64_t* fl, const double* fw, int32_t nf,
+                      int32_t wf, const int64_t* e, const int64_t* el, const double* ew,
+                      int32_t ne, int32_t we) override {
+    if (nf < 0 || ne < 0 || wf < 0 || we < 0)
+      return fail(GEO_ERR_INVALID, "H receiver counts and widths must be non-negative");
+    const int64_t kf = (int64_t)nf * std::max(wf, 0), ke = (int64_t)ne * std::max(we, 0);
+    std::vector<int32_t> idx(std::max<int64_t>(2 * (kf + ke), 1), 0);
+    std::vector<C> w(std::max<int64_t>(kf + ke, 1), 0);
+    for (int64_t k = 0; k < kf; ++k) {
+      if (f[k] < 0 || f[k] >= F) return fail(GEO_ERR_INVALID, "observation face index is out of range");
+      if (fl[k] < 0 || fl[k] >= R)
+        return fail(GEO_ERR_INVALID, "radial H observation layer is out of range");
+      idx[k] = (int32_t)f[k];
+      idx[kf + k] = (int32_t)fl[k];
+      w[k] = (C)fw[k];
+    }
+    for (int64_t k = 0; k < ke; ++k) {
+      if (e[k] < 0 || e[k] >= E) return fail(GEO_ERR_INVALID, "observation edge index is out of range");
+      if (el[k] < 0 || el[k] > R)
+        return fail(GEO_ERR_INVALID, "tangential H observation layer is out of range");
+      idx[2 * kf + k] = (int32_t)e[k];
+      idx[2 * kf + ke + k] = (int32_t)el[k];
+      w[kf + k] = (C)ew[k];
+    }
+    drop_graph();
+    if (hrx_mem) (void)hipFree(hrx_mem);
+    hrx_mem = nullptr;
+    const size_t bi = idx.size() * sizeof(int32_t);
+    const size_t off_w = round_up((int64_t)bi, 16);
+    GEO_HIP(hipMalloc(&hrx_mem, off_w + w.size() * sizeof(C)));
+    char* base = static_cast<char*>(hrx_mem);
+    GEO_HIP(hipMemcpy(base, idx.data(), bi, hipMemcpyHostToDevice));
+    GEO_HIP(hipMemcpy(base + off_w, w.data(), w.size() * sizeof(C), hipMemcpyHostToDevice));
+    const int32_t* di = reinterpret_cast<const int32_t*>(base);
+    const C* dw = reinterpret_cast<const C*>(base + off_w);
+    ep.hrx_face = di;
+    ep.hrx_flayer = di + kf;
+    ep.hrx_edge = di + 2 * kf;
+    ep.hrx_elayer = di + 2 * kf + ke;
+    ep.hrx_fw = dw;
+    ep.hrx_ew = dw + kf;
+    ep.n_hrx_f = nf;
+    ep.hrx_fwidth = wf;
+    ep.n_hrx_e = ne;
+    ep.hrx_ewidth = we;
+    ++generation;
+    return GEO_OK;
+  }
+
   void launch_h(hipStream_t s) {
     if (plasma) {
       if (magnetized) hipLaunchKernelGGL((geo::h_step_kernel<T, C, true, true>), dim3(h_grid), dim3(geo::kBlock), 0, s, hp);
@@ -1241,6 +1301,22 @@ struct Plan final : PlanBase {
     return ensure_graph(chunk);
   }
 
+  int step_h(const void* currents, int64_t n, int32_t chunk, int32_t every, void* rt, void* tt,
+             hipStream_t stream) override {
+    if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
+    if (n < 0) return fail(GEO_ERR_INVALID, "step count must be non-negative");
+    if (every < 1) return fail(GEO_ERR_INVALID, "sample_every must be positive");
+    if ((rt != nullptr && ep.n_hrx_f <= 0) || (tt != nullptr && ep.n_hrx_e <= 0))
+      return fail(GEO_ERR_INVALID, "H trace recording requires H receivers");
+    hipLaunchKernelGGL(geo::set_dynamic_kernel, dim3(1), dim3(1), 0, stream, dyn, (int64_t)0,
+                       (void*)nullptr, (int64_t)0, currents, n);
+    const int64_t rows = n / every + 1 + (n % every != 0 ? 1 : 0);
+    hipLaunchKernelGGL((geo::set_h_trace_kernel<T, C>), dim3(1), dim3(geo::kBlock), 0, stream, ep,
+                       rt, tt, rows, (int64_t)every, n);
+    GEO_HIP(hipGetLastError());
+    return run_steps(n, chunk, stream);
+  }
+
   int step(const void* currents, int64_t n, int32_t chunk, void* trace,
            hipStream_t stream) override {
     if (!bound) return fail(GEO_ERR_INVALID, "plan has no bound state");
@@ -1250,6 +1326,18 @@ struct Plan final : PlanBase {
     hipLaunchKernelGGL(geo::set_dynamic_kernel, dim3(1), dim3(1), 0, stream, dyn, (int64_t)0,
                        trace, trace ? n : (int64_t)0, currents, n);
     GEO_HIP(hipGetLastError());
+    const int rc = run_steps(n, chunk, stream);
+    if (rc != GEO_OK) return rc;
+    if (trace != nullptr) {
+      C* last = static_cast<C*>(trace) + n * n_rx;
+      return sample(last, stream);
+    }
+    return GEO_OK;
+  }
+
+  // n steps after the device block was set for the call: full chunks as
+  // graph replays, the rest as single launches.
+  int run_steps(int64_t n, int32_t chunk, hipStream_t stream) {
     // rescale points (narrow storage): every `interval` steps from the start
     // of the call, identical with and without graphs for the same chunk
     const int32_t interval = chunk > 1 ? chunk : kRescaleInterval;
@@ -1265,10 +1353,6 @@ struct Plan final : PlanBase {
       launch_e(stream);
     }
     GEO_HIP(hipGetLastError());
-    if (trace != nullptr) {
-      C* last = static_cast<C*>(trace) + n * n_rx;
-      return sample(last, stream);
-    }
     return GEO_OK;
   }
 
@@ -2157,6 +2241,23 @@ int geo_set_receivers(GeoPlan* plan, const int64_t* vertices, const int64_t* lay
   if (n_receivers > 0 && (!vertices || !layers || !weights))
     return fail(GEO_ERR_INVALID, "null receiver tables");
   return plan->impl->set_receivers(vertices, layers, weights, n_receivers);
+}
+
+int geo_set_h_receivers(GeoPlan* plan, const int64_t* faces, const int64_t* face_layers,
+                        const double* face_weights, int32_t n_face_receivers, int32_t face_width,
+                        const int64_t* edges, const int64_t* edge_layers,
+                        const double* edge_weights, int32_t n_edge_receivers, int32_t edge_width) {
+  if (!plan) return fail(GEO_ERR_INVALID, "null plan");
+  return plan->impl->set_h_receivers(faces, face_layers, face_weights, n_face_receivers,
+                                     face_width, edges, edge_layers, edge_weights,
+                                     n_edge_receivers, edge_width);
+}
+
+int geo_step_h(GeoPlan* plan, const void* currents, int64_t n_steps, int32_t chunk,
+               int32_t sample_every, void* hr_trace, void* ht_trace, void* stream) {
+  if (!plan) return fail(GEO_ERR_INVALID, "null plan");
+  return plan->impl->step_h(currents, n_steps, chunk, sample_every, hr_trace, ht_trace,
+                            static_cast<hipStream_t>(stream));
 }
 
 int geo_sample_er(GeoPlan* plan, void* row, void* stream) {

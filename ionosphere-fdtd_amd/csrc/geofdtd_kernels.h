@@ -36,7 +36,7 @@ constexpr int kNumXcd = 8;
 
 enum Segment : uint32_t {
   kSegHEdge = 0, kSegHFace = 1, kSegSample = 3,
-  kSegEVertex = 4, kSegEEdge = 5,
+  kSegEVertex = 4, kSegEEdge = 5, kSegHSample = 6,
 };
 
 // Column segment geometry: an integer number of columns per 256-thread
@@ -71,6 +71,16 @@ struct Dynamic {
   // the same launch writes.
   uint32_t sr_h;
   uint32_t sr_e;
+  // H traces (record_h_observations): (h_rows, n) compute-type rows of the
+  // weighted Hr / Ht samples, or null.  The E launch of the step that
+  // completes step c (counter == c) writes row c / h_every when h_every
+  // divides c, and row h_total / h_every + 1 after the last step h_total
+  // otherwise; row 0 (the state before the call) is written separately.
+  void* h_trace_r;
+  void* h_trace_t;
+  int64_t h_rows;
+  int64_t h_every;
+  int64_t h_total;
 };
 
 enum : int32_t { kFieldEr = 0, kFieldEt = 1, kFieldHr = 2, kFieldHt = 3 };
@@ -507,6 +517,15 @@ struct EParams {
   int32_t tsrc_row[kMaxSource], tsrc_layer[kMaxSource];
   C tsrc_factor[kMaxSource];
   Dynamic* dyn;
+  // H receivers: (n, width) faces with midpoint layers, edges with node
+  // layers, and their weights (record_h_observations, solver.py:1138-1229)
+  const int32_t* __restrict__ hrx_face;
+  const int32_t* __restrict__ hrx_flayer;
+  const C* __restrict__ hrx_fw;
+  const int32_t* __restrict__ hrx_edge;
+  const int32_t* __restrict__ hrx_elayer;
+  const C* __restrict__ hrx_ew;
+  int32_t n_hrx_f, hrx_fwidth, n_hrx_e, hrx_ewidth;
   const uint32_t* __restrict__ block_map;
   int32_t n_blocks, per_xcd;
   SegGeom g_vertex, g_edge;
@@ -1260,9 +1279,57 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
   store_strip(erow, et, sc.et, sc.rd_et, (uint32_t)e * (uint32_t)p.pm + (uint32_t)j0);
 }
 
+// Weighted H samples of one trace row: sum_q w[i,q] * H[entity[i,q], layer[i,q]]
+// in the order of the reference's (selected * weights).sum(axis=1).  The E
+// launch does not write H, so any of its blocks sees the H state after this
+// step's H half-step.
+template <typename T, typename C>
+__device__ __forceinline__ void h_sample_row(const EParams<T, C>& p, const Scales<C>& sc,
+                                             C* rt, C* tt, int64_t row, int64_t rows) {
+  if (row < 0 || row >= rows) return;
+  if (rt != nullptr)
+    for (int i = threadIdx.x; i < p.n_hrx_f; i += blockDim.x) {
+      C acc = (C)0;
+      for (int q = 0; q < p.hrx_fwidth; ++q) {
+        const int32_t k = i * p.hrx_fwidth + q;
+        acc += load_one(p.hr + (size_t)p.hrx_face[k] * p.pm + p.hrx_flayer[k], sc.hr_inv) *
+               p.hrx_fw[k];
+      }
+      rt[row * p.n_hrx_f + i] = acc;
+    }
+  if (tt != nullptr)
+    for (int i = threadIdx.x; i < p.n_hrx_e; i += blockDim.x) {
+      C acc = (C)0;
+      for (int q = 0; q < p.hrx_ewidth; ++q) {
+        const int32_t k = i * p.hrx_ewidth + q;
+        acc += load_one(p.ht + (size_t)p.hrx_edge[k] * p.pn + p.hrx_elayer[k], sc.ht_inv) *
+               p.hrx_ew[k];
+      }
+      tt[row * p.n_hrx_e + i] = acc;
+    }
+}
+
+// The E launch's H-sampling block: the counter was advanced by this step's
+// H launch, so it counts the steps completed with this one.
+template <typename T, typename C>
+__device__ __forceinline__ void h_sample_block(const EParams<T, C>& p, const Scales<C>& sc) {
+  const Dynamic* d = p.dyn;
+  if (d->h_trace_r == nullptr && d->h_trace_t == nullptr) return;
+  const int64_t c = d->counter;
+  const int64_t row = c % d->h_every == 0 ? c / d->h_every
+                      : c == d->h_total    ? d->h_total / d->h_every + 1
+                                           : -1;
+  h_sample_row(p, sc, reinterpret_cast<C*>(d->h_trace_r), reinterpret_cast<C*>(d->h_trace_t),
+               row, d->h_rows);
+}
+
 template <typename T, typename C, bool PLASMA, bool MAGNETIZED, int S>
 __device__ __forceinline__ void e_dispatch(const EParams<T, C>& p, const Scales<C>& sc,
                                            uint32_t seg, int32_t b) {
+  if (seg == kSegHSample) {
+    h_sample_block(p, sc);
+    return;
+  }
   int32_t row, col;
   if constexpr (is_narrow<T>()) {
     // the rounding step index of the next H launch: written by one thread,
@@ -1363,6 +1430,9 @@ __global__ void set_dynamic_runs_kernel(Dynamic* const* __restrict__ dyns, int32
   d->n_currents = currents != nullptr ? n_currents : 0;
   d->trace = trace != nullptr ? trace + i * trace_stride : nullptr;
   d->trace_rows = trace != nullptr ? rows : 0;
+  d->h_trace_r = nullptr;
+  d->h_trace_t = nullptr;
+  d->h_rows = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1376,6 +1446,28 @@ __global__ void set_dynamic_kernel(Dynamic* dyn, int64_t counter, void* trace, i
   dyn->trace_rows = rows;
   dyn->currents = currents;
   dyn->n_currents = currents != nullptr ? n_currents : 0;
+  dyn->h_trace_r = nullptr;
+  dyn->h_trace_t = nullptr;
+  dyn->h_rows = 0;
+  dyn->h_every = 1;
+  dyn->h_total = 0;
+}
+
+// H traces of the call that follows (after set_dynamic_kernel), and row 0:
+// the H state before its first step.
+template <typename T, typename C>
+__global__ void __launch_bounds__(kBlock) set_h_trace_kernel(const EParams<T, C> p, void* rt,
+                                                            void* tt, int64_t rows, int64_t every,
+                                                            int64_t total) {
+  if (threadIdx.x == 0) {
+    p.dyn->h_trace_r = rt;
+    p.dyn->h_trace_t = tt;
+    p.dyn->h_rows = rows;
+    p.dyn->h_every = every;
+    p.dyn->h_total = total;
+  }
+  h_sample_row(p, load_scales<T, C, true>(p.dyn), static_cast<C*>(rt), static_cast<C*>(tt),
+               0, rows);
 }
 
 template <typename T, typename C>

@@ -818,29 +818,38 @@ class GeodesicFDTD:
             raise ValueError("tangential H observation layer is out of range")
         if not np.all(np.isfinite(fw)) or not np.all(np.isfinite(ew)):
             raise ValueError("observation weights must be finite")
+        n_f, n_e = faces.shape[0], edges.shape[0]
+        key = tuple(np.ascontiguousarray(a).tobytes() for a in (faces, flayers, fw, edges,
+                                                                 elayers, ew)) + (
+            faces.shape, edges.shape)
+        if key != getattr(self, "_h_receiver_key", None):
+            tables = [np.ascontiguousarray(a, dtype=t) for a, t in (
+                (faces, np.int64), (flayers, np.int64), (fw, np.float64),
+                (edges, np.int64), (elayers, np.int64), (ew, np.float64))]
+            _native.check(self._lib.geo_set_h_receivers(
+                self._plan, _native.i64_ptr(tables[0]), _native.i64_ptr(tables[1]),
+                _native.f64_ptr(tables[2]), n_f, faces.shape[1],
+                _native.i64_ptr(tables[3]), _native.i64_ptr(tables[4]),
+                _native.f64_ptr(tables[5]), n_e, edges.shape[1]), self._lib)
+            self._h_receiver_key = key
         torch = self.backend.torch
-        dev = self.backend.torch_device
-        cdt = self.backend.compute_dtype
-        tf, tfl, tfw = (torch.as_tensor(faces, device=dev), torch.as_tensor(flayers, device=dev),
-                        torch.as_tensor(fw, dtype=cdt, device=dev))
-        te, tel, tew = (torch.as_tensor(edges, device=dev), torch.as_tensor(elayers, device=dev),
-                        torch.as_tensor(ew, dtype=cdt, device=dev))
-        sample_steps = np.unique(np.concatenate((np.arange(0, steps + 1, sample_every),
-                                                 np.asarray((steps,)))))
-        rt = torch.zeros((len(sample_steps), faces.shape[0]), dtype=cdt, device=dev)
-        tt = torch.zeros((len(sample_steps), edges.shape[0]), dtype=cdt, device=dev)
-
-        def sample(row: int) -> None:
-            hr = self.hr
-            ht = self.ht
-            rt[row] = (hr[tf, tfl].to(cdt) * tfw).sum(dim=1)
-            tt[row] = (ht[te, tel].to(cdt) * tew).sum(dim=1)
-
-        start = self.steps
-        sample(0)
-        for row, target in enumerate(sample_steps[1:], start=1):
-            self.step(int(target) + start - self.steps)
-            sample(row)
+        rows = steps // sample_every + 1 + (1 if steps % sample_every else 0)
+        rt = torch.zeros((rows, n_f), dtype=self.backend.compute_dtype,
+                         device=self.backend.torch_device)
+        tt = torch.zeros((rows, n_e), dtype=self.backend.compute_dtype,
+                         device=self.backend.torch_device)
+        # One device call: row 0 before the first step, then one row from the
+        # E launch of every sampled step (graph chunks included).
+        self._sync_step_index(steps)
+        currents = self._source_currents(steps)
+        chunk = self.compile_chunk_size if self.compiled else 1
+        _native.check(self._lib.geo_step_h(
+            self._plan, None if currents is None else currents.data_ptr(), steps, chunk,
+            sample_every, rt.data_ptr() if n_f else None, tt.data_ptr() if n_e else None,
+            self._stream()), self._lib)
+        self._pending = currents
+        self.steps += steps
+        self.time_s = self.steps * self.time_step_s
         self.backend.synchronize()
         return self.to_numpy(rt), self.to_numpy(tt)
 

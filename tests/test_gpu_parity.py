@@ -396,3 +396,51 @@ def test_h_observations_match_oracle_fp64():
     np.testing.assert_allclose(got_r, want_r, rtol=0, atol=FP64_BAR * scale)
     np.testing.assert_allclose(got_t, want_t, rtol=0, atol=FP64_BAR * scale)
     assert sim.steps == ref.steps == 100
+
+
+@pytest.mark.parametrize("steps,every,compiled", [(40, 1, True), (37, 5, True), (4, 9, False),
+                                                  (0, 2, False)])
+def test_h_observations_device_path_cases(steps, every, compiled):
+    """The H traces written by the E launch's sampling block: every step
+    through graph chunks, a last row that sample_every does not divide,
+    sample_every beyond the step count, and no step at all; one-entity rows
+    (width 1) and no edge receivers beside the face ones."""
+    sc = S.scenario("plasma_random")
+    sim = S.build_sim(sc, compile_step=compiled, compile_chunk_size=8)
+    ref = S.build_oracle(sc)
+    _seed_fields(sim, ref)  # H at O(1e-6) from the first row
+    mesh = sim.mesh
+    rng = np.random.default_rng(5)
+    R = len(sim.radial_midpoints_m)
+    faces = rng.integers(0, mesh.n_faces, (6, 1))
+    flayers = rng.integers(0, R, (6, 1))
+    fw = rng.random((6, 1))
+    none_e = np.zeros((0, 2), dtype=np.int64)
+    got_r, got_t = sim.record_h_observations(faces, flayers, fw, none_e, none_e,
+                                             np.zeros((0, 2)), steps, sample_every=every)
+    want_r, want_t = ref.record_h_observations(faces, flayers, fw, none_e, none_e,
+                                               np.zeros((0, 2)), steps, sample_every=every)
+    assert got_r.shape == want_r.shape and got_t.shape == want_t.shape == (want_r.shape[0], 0)
+    scale = float(np.max(np.abs(want_r)))
+    assert scale > 1e-9
+    np.testing.assert_allclose(got_r, want_r, rtol=0, atol=FP64_BAR * scale)
+    # a second call starts from the state the first one left: its row 0 is
+    # the first call's last row (the oracle, like the reference, samples from
+    # step 0 only: solver.py:1220 steps to target_step - self.steps)
+    edges = rng.integers(0, mesh.n_edges, (3, 4))
+    elayers = rng.integers(0, R + 1, (3, 4))
+    ew = rng.random((3, 4))
+    again_r, again_t = sim.record_h_observations(faces, flayers, fw, edges, elayers, ew, 12,
+                                                 sample_every=every)
+    assert again_r[0].tobytes() == got_r[-1].tobytes()
+    assert again_t.shape == (12 // every + 1 + (1 if 12 % every else 0), 3)
+    ref.step(12)
+    want_t_last = (ref.ht[edges, elayers] * ew).sum(axis=1)
+    np.testing.assert_allclose(again_t[-1], want_t_last, rtol=0,
+                               atol=FP64_BAR * float(np.max(np.abs(want_t_last))))
+    assert sim.steps == ref.steps == steps + 12
+    # the Er trace path afterwards carries no H rows into stale buffers
+    v, l, w = S.receiver_tables(sim.mesh, sim.altitudes_m, S.receiver_points(sc))
+    np.testing.assert_allclose(sim.record_er_observations(v, l, w, 10),
+                               ref.record_er_observations(v, l, w, 10), rtol=0,
+                               atol=FP64_BAR * float(np.max(np.abs(ref.er))))
