@@ -11,7 +11,7 @@ gfx950 kernels in ``libgeofdtd.so`` (C ABI in include/geofdtd.h).
 from .backends import ArrayBackend, BackendUnavailableError, create_backend
 from .constants import C_0, EARTH_RADIUS_M, EPSILON_0, MU_0
 from .ensemble import EnsembleFDTD
-from .materials import EarthIonosphereMaterial
+from .materials import EarthIonosphereMaterial, SphericalAnomaly
 from .mesh import GeodesicMesh, build_geodesic_mesh, build_geodesic_mesh_from_topology
 from .physics_diagnostics import PhysicsDiagnosticSampler, PhysicsSnapshot
 from .plasma import ColdPlasmaSpecies, MeshPlasmaModel, electron_drude_model
@@ -27,7 +27,7 @@ __all__ = [
     "EnsembleFDTD",
     "GaussianCurrent", "GeodesicFDTD", "GeodesicMesh", "MU_0", "MeshPlasmaModel",
     "PhysicsDiagnosticSampler", "PhysicsSnapshot",
-    "RadialGridValidation", "RadialRefinementRegion", "SimulationConfig",
+    "RadialGridValidation", "RadialRefinementRegion", "SimulationConfig", "SphericalAnomaly",
     "TangentialGaussianCurrent", "build_geodesic_mesh", "build_geodesic_mesh_from_topology",
     "build_refined_radial_grid", "create_backend", "electron_drude_model",
     "validate_radial_grid",

@@ -92,6 +92,46 @@ class GeodesicMesh:
         right = self.face_centers[self.edge_right_faces]
         return _signed_area(tail, head, left) - _signed_area(tail, head, right)
 
+    def edge_diamond_quadrant_solid_angles(self) -> FloatArray:
+        """(E, 4) areas of the four spherical triangles that split an edge's
+        diamond at its midpoint m: (m, tail, left), (m, left, head),
+        (m, head, right), (m, right, tail), with left / right the adjacent face
+        centres (mesh.py:106-121).  They sum to edge_diamond_solid_angles."""
+        tail = self.vertices[self.edges[:, 0]]
+        head = self.vertices[self.edges[:, 1]]
+        mid = self.edge_midpoints()
+        left = self.face_centers[self.edge_left_faces]
+        right = self.face_centers[self.edge_right_faces]
+        corners = ((tail, left), (left, head), (head, right), (right, tail))
+        return np.stack([-_signed_area(mid, a, b) for a, b in corners], axis=1)
+
+    def dual_cell_wedge_quadrature(self, vertex_indices, edge_indices
+                                   ) -> tuple[FloatArray, FloatArray]:
+        """One quadrature point per dual-cell wedge (mesh.py:123-155).
+
+        The wedge of vertex v and incident edge e is the spherical triangle
+        (v, left centre, right centre) of e; the wedges of a vertex tile its
+        dual cell.  Returns the normalised vertex-corner sum as the direction
+        and the wedge's area, oriented so that it is positive for both ends of
+        the edge.
+        """
+        v = np.asarray(vertex_indices, dtype=np.int64)
+        e = np.asarray(edge_indices, dtype=np.int64)
+        if v.ndim != 1 or e.shape != v.shape:
+            raise ValueError("dual-cell wedge indices must be matching 1-D arrays")
+        if np.any((v < 0) | (v >= self.n_vertices)):
+            raise ValueError("dual-cell wedge vertex index is out of bounds")
+        if np.any((e < 0) | (e >= self.n_edges)):
+            raise ValueError("dual-cell wedge edge index is out of bounds")
+        ends = self.edges[e]
+        if not np.all((ends[:, 0] == v) | (ends[:, 1] == v)):
+            raise ValueError("dual-cell wedge vertex must be incident to its edge")
+        corner = self.vertices[v]
+        left = self.face_centers[self.edge_left_faces[e]]
+        right = self.face_centers[self.edge_right_faces[e]]
+        area = _signed_area(corner, left, right)
+        return _normalize(corner + left + right), np.where(ends[:, 1] == v, area, -area)
+
     # Host operators with the reference's semantics (mesh.py:157-188); used by
     # setup code and tests, never by the time step.
     def face_circulation(self, edge_values: FloatArray) -> FloatArray:

@@ -19,6 +19,7 @@ goes through the setters.
 from __future__ import annotations
 
 import ctypes as C
+import dataclasses
 from dataclasses import dataclass
 from pathlib import Path
 from typing import Any
@@ -29,7 +30,8 @@ from numpy.typing import NDArray
 from . import _native
 from .backends import ArrayBackend, BackendUnavailableError, create_backend
 from .constants import C_0, EARTH_RADIUS_M, EPSILON_0, MU_0
-from .materials import EarthIonosphereMaterial
+from .materials import (EarthIonosphereMaterial, blend_cell_anomalies, blend_node_anomalies,
+                        conservative_anomaly_fractions)
 from .mesh import GeodesicMesh, build_geodesic_mesh, vertex_incidence
 from .plasma import MeshPlasmaModel, plasma_tables
 from .radial_grid import validate_radial_grid
@@ -217,9 +219,6 @@ class GeodesicFDTD:
         self.compile_chunk_size = int(compile_chunk_size)
         self.config = config or SimulationConfig()
         cfg = self.config
-        if cfg.radial_material_support != "point" or cfg.tangential_material_support != "point" \
-                or cfg.horizontal_anomaly_mode != "point":
-            raise ValueError("this package supports point material sampling only")
         if mesh is None:
             self.mesh = build_geodesic_mesh(cfg.subdivision, cfg.mesh_relaxations,
                                             cfg.mesh_orientation, cfg.mesh_optimization_steps)
@@ -265,10 +264,16 @@ class GeodesicFDTD:
         self.radial_node_control_lengths_m = ctrl
         if plasma is not None:
             plasma.validate_grid(self.mesh, self.radial_midpoint_altitudes_m)
-        (self.sigma_er, self.epsilon_r_er, self.sigma_et,
-         self.epsilon_r_et) = sample_material(self.material, self.mesh, self.altitudes_m,
-                                              self.radial_midpoint_altitudes_m,
-                                              cfg.earth_radius_m)
+        sampled = sample_material(self.material, self.mesh, self.altitudes_m,
+                                  self.radial_midpoint_altitudes_m, cfg.earth_radius_m,
+                                  radial_support=cfg.radial_material_support,
+                                  tangential_support=cfg.tangential_material_support,
+                                  anomaly_mode=cfg.horizontal_anomaly_mode)
+        self.sigma_er, self.epsilon_r_er = sampled.sigma_er, sampled.epsilon_r_er
+        self.sigma_et, self.epsilon_r_et = sampled.sigma_et, sampled.epsilon_r_et
+        if sampled.fractions_er is not None:
+            self.anomaly_horizontal_fractions_er = sampled.fractions_er
+            self.anomaly_horizontal_fractions_et = sampled.fractions_et
         self.cfl_time_step_limit_s = cfl_time_step_limit(
             self.mesh, self.radii_m, self.radial_steps_m, self.epsilon_r_er, self.epsilon_r_et,
             plasma)
@@ -1008,29 +1013,147 @@ def _validated_material(values, shape, label):
     return sigma, eps
 
 
-def sample_material(material, mesh, altitudes_m, midpoint_altitudes_m, earth_radius_m):
-    """Point-sampled (sigma_er, eps_r_er, sigma_et, eps_r_et) (solver.py:467-629).
+@dataclass
+class MaterialSample:
+    """Material on the staggered grid: Er on vertices x radial nodes, Et on
+    edges x radial midpoints, and the horizontal anomaly fractions of the
+    conservative-nearest mode (None otherwise)."""
 
-    Er samples sit on vertices x radial nodes, Et samples on edge midpoints x
-    radial midpoints; a material with a ``sample_mesh`` hook provides all four.
+    sigma_er: NDArray[np.float64]
+    epsilon_r_er: NDArray[np.float64]
+    sigma_et: NDArray[np.float64]
+    epsilon_r_et: NDArray[np.float64]
+    fractions_er: tuple | None = None
+    fractions_et: tuple | None = None
+
+
+# wedges sampled per call of a material's sampler in the dual-cell average
+_WEDGES_PER_SAMPLE = 65_536
+
+
+def _dual_cell_average(material, mesh, altitudes_m, earth_radius_m):
+    """Er material averaged over each polygonal dual cell (solver.py:631-705):
+    one quadrature point per (vertex, incident edge) wedge, weighted by the
+    wedge's share of the cell area, summed per vertex in edge order."""
+    ends = mesh.edges.ravel()
+    order = np.argsort(ends, kind="stable")
+    wv, we = ends[order], order // 2
+    directions, area = mesh.dual_cell_wedge_quadrature(wv, we)
+    weight = area / mesh.dual_cell_solid_angles[wv]
+    if not np.allclose(np.bincount(wv, weights=weight, minlength=mesh.n_vertices), 1.0,
+                       rtol=0.0, atol=2.0e-12):
+        raise RuntimeError("dual-cell material weights do not close")
+    shape = (mesh.n_vertices, len(altitudes_m))
+    sigma, eps = np.zeros(shape), np.zeros(shape)
+    for lo in range(0, len(wv), _WEDGES_PER_SAMPLE):
+        part = slice(lo, lo + _WEDGES_PER_SAMPLE)
+        s, e = _validated_material(material.sample(directions[part], altitudes_m, earth_radius_m),
+                                   (len(directions[part]), len(altitudes_m)),
+                                   "radial dual-cell support")
+        np.add.at(sigma, wv[part], weight[part, None] * s)
+        np.add.at(eps, wv[part], weight[part, None] * e)
+    return sigma, eps
+
+
+def _sample_tangential(material, directions, altitudes_m, midpoint_altitudes_m, earth_radius_m):
+    """Et material at unit directions: a material's radial-cell sampler when
+    it has one (sample_tangential_cells over [alt_j, alt_j+1]), else point
+    samples at the radial midpoints."""
+    cells = getattr(material, "sample_tangential_cells", None)
+    if cells is None:
+        values = material.sample(directions, midpoint_altitudes_m, earth_radius_m)
+    else:
+        values = cells(directions, altitudes_m[:-1], altitudes_m[1:], earth_radius_m)
+    return _validated_material(values, (len(directions), len(midpoint_altitudes_m)),
+                               "tangential")
+
+
+def _edge_diamond_average(material, mesh, altitudes_m, midpoint_altitudes_m, earth_radius_m):
+    """Et material averaged over each edge diamond (solver.py:579-606): one
+    quadrature point per quarter-diamond triangle, weighted by its area."""
+    mid = mesh.edge_midpoints()
+    tail, head = mesh.vertices[mesh.edges[:, 0]], mesh.vertices[mesh.edges[:, 1]]
+    left = mesh.face_centers[mesh.edge_left_faces]
+    right = mesh.face_centers[mesh.edge_right_faces]
+    quadrants = mesh.edge_diamond_quadrant_solid_angles()
+    share = quadrants / np.sum(quadrants, axis=1, keepdims=True)
+    shape = (mesh.n_edges, len(midpoint_altitudes_m))
+    sigma, eps = np.zeros(shape), np.zeros(shape)
+    for k, (a, b) in enumerate(((tail, left), (left, head), (head, right), (right, tail))):
+        d = mid + a + b
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        s, e = _sample_tangential(material, d, altitudes_m, midpoint_altitudes_m, earth_radius_m)
+        sigma += share[:, k, None] * s
+        eps += share[:, k, None] * e
+    return sigma, eps
+
+
+def sample_material(material, mesh, altitudes_m, midpoint_altitudes_m, earth_radius_m, *,
+                    radial_support="point", tangential_support="point",
+                    anomaly_mode="point") -> MaterialSample:
+    """Material on the staggered grid (solver.py:467-629).
+
+    * Er: point samples at vertices x radial nodes, or (radial_support
+      "dual-cell") the dual-cell wedge average;
+    * Et: samples at edge midpoints x radial midpoints, or
+      (tangential_support "edge-diamond") the quarter-diamond average;
+    * anomaly_mode "conservative-nearest": the material's spherical anomalies
+      are taken out of point sampling and rasterised onto the dual cells and
+      edge diamonds with their exact areas instead.
+    A material with a ``sample_mesh`` hook provides all four arrays itself.
     """
+    altitudes_m = np.asarray(altitudes_m, dtype=np.float64)
+    midpoint_altitudes_m = np.asarray(midpoint_altitudes_m, dtype=np.float64)
     shape_er = (mesh.n_vertices, len(altitudes_m))
     shape_et = (mesh.n_edges, len(midpoint_altitudes_m))
     sampler = getattr(material, "sample_mesh", None)
     if sampler is not None:
-        values = sampler(mesh, altitudes_m, earth_radius_m, radial_material_support="point",
-                         tangential_material_support="point", horizontal_anomaly_mode="point")
+        values = sampler(mesh, altitudes_m, earth_radius_m, radial_material_support=radial_support,
+                         tangential_material_support=tangential_support,
+                         horizontal_anomaly_mode=anomaly_mode)
         try:
             s_er, e_er, s_et, e_et = values
         except (TypeError, ValueError) as error:
             raise ValueError("mesh material sampler must return four property arrays") \
                 from error
-        return (*_validated_material((s_er, e_er), shape_er, "mesh-native radial"),
-                *_validated_material((s_et, e_et), shape_et, "mesh-native tangential"))
-    return (*_validated_material(material.sample(mesh.vertices, altitudes_m, earth_radius_m),
-                                 shape_er, "radial"),
-            *_validated_material(material.sample(mesh.edge_midpoints(), midpoint_altitudes_m,
-                                                 earth_radius_m), shape_et, "tangential"))
+        return MaterialSample(*_validated_material((s_er, e_er), shape_er, "mesh-native radial"),
+                              *_validated_material((s_et, e_et), shape_et,
+                                                   "mesh-native tangential"))
+    anomalies = getattr(material, "anomalies", None)
+    if anomaly_mode == "conservative-nearest" and anomalies is None:
+        raise ValueError("conservative-nearest anomalies require a material with an "
+                         "anomalies collection")
+    conservative = anomaly_mode == "conservative-nearest" and bool(anomalies)
+    base = material
+    if conservative:
+        try:
+            base = dataclasses.replace(material, anomalies=())
+        except TypeError as error:
+            raise ValueError("conservative-nearest anomalies require a dataclass material") \
+                from error
+    if radial_support == "point":
+        s_er, e_er = _validated_material(base.sample(mesh.vertices, altitudes_m, earth_radius_m),
+                                         shape_er, "radial")
+    else:
+        s_er, e_er = _dual_cell_average(base, mesh, altitudes_m, earth_radius_m)
+    if tangential_support == "point":
+        s_et, e_et = _sample_tangential(base, mesh.edge_midpoints(), altitudes_m,
+                                        midpoint_altitudes_m, earth_radius_m)
+    else:
+        s_et, e_et = _edge_diamond_average(base, mesh, altitudes_m, midpoint_altitudes_m,
+                                           earth_radius_m)
+    out = MaterialSample(s_er, e_er, s_et, e_et)
+    if conservative:
+        anomalies = tuple(anomalies)
+        out.fractions_er = tuple(conservative_anomaly_fractions(
+            mesh.vertices, mesh.dual_cell_solid_angles, a, earth_radius_m) for a in anomalies)
+        blend_node_anomalies(s_er, e_er, altitudes_m, anomalies, out.fractions_er)
+        out.fractions_et = tuple(conservative_anomaly_fractions(
+            mesh.edge_midpoints(), mesh.edge_diamond_solid_angles(), a, earth_radius_m)
+            for a in anomalies)
+        blend_cell_anomalies(s_et, e_et, altitudes_m[:-1], altitudes_m[1:], anomalies,
+                             out.fractions_et)
+    return out
 
 
 def cfl_time_step_limit(mesh, radii_m, radial_steps_m, epsilon_r_er, epsilon_r_et,
@@ -1061,9 +1184,12 @@ def stable_time_step(config: SimulationConfig, mesh, material=None, plasma=None)
         altitudes = np.asarray(config.radial_altitudes_m, dtype=np.float64)
     radii = config.earth_radius_m + altitudes
     mids = 0.5 * (radii[:-1] + radii[1:])
-    _, e_er, _, e_et = sample_material(material, mesh, altitudes, mids - config.earth_radius_m,
-                                       config.earth_radius_m)
-    limit = cfl_time_step_limit(mesh, radii, np.diff(radii), e_er, e_et, plasma)
+    m = sample_material(material, mesh, altitudes, mids - config.earth_radius_m,
+                        config.earth_radius_m, radial_support=config.radial_material_support,
+                        tangential_support=config.tangential_material_support,
+                        anomaly_mode=config.horizontal_anomaly_mode)
+    limit = cfl_time_step_limit(mesh, radii, np.diff(radii), m.epsilon_r_er, m.epsilon_r_et,
+                                plasma)
     return config.time_step_s if config.time_step_s is not None else config.courant_factor * limit
 
 

@@ -444,3 +444,49 @@ def test_h_observations_device_path_cases(steps, every, compiled):
     np.testing.assert_allclose(sim.record_er_observations(v, l, w, 10),
                                ref.record_er_observations(v, l, w, 10), rtol=0,
                                atol=FP64_BAR * float(np.max(np.abs(ref.er))))
+
+
+def test_cell_averaged_materials_with_anomalies_match_oracle_fp64():
+    """Dual-cell Er and edge-diamond Et materials with conservatively
+    rasterised crust anomalies (horizontally varying coefficients on every
+    row: the per-entity coefficient path of both kernels), seeded fields, a
+    ground-level source: FP64 HIP against the oracle fed the same sampled
+    material arrays.  The sampling itself is pinned on the CPU by the
+    reference's vectors (tests/test_materials.py)."""
+    import ionosphere_fdtd_amd as g
+
+    o = S.oracle_module()
+    anomalies = (
+        g.SphericalAnomaly(latitude_deg=35.15, longitude_deg=126.85, radius_m=900e3,
+                           altitude_min_m=-15e3, altitude_max_m=-2e3, conductivity_factor=40.0,
+                           relative_permittivity=25.0, maximum_background_conductivity_s_m=5e-3),
+        g.SphericalAnomaly(latitude_deg=-20.0, longitude_deg=-60.0, radius_m=1500e3,
+                           altitude_min_m=-20e3, altitude_max_m=5e3, conductivity_factor=0.1,
+                           relative_permittivity=4.0, target_area_m2=5.0e12))
+    cfg = g.SimulationConfig(subdivision=2, radial_cells=12, minimum_altitude_m=-20e3,
+                             maximum_altitude_m=100e3, radial_material_support="dual-cell",
+                             tangential_material_support="edge-diamond",
+                             horizontal_anomaly_mode="conservative-nearest")
+    material = g.EarthIonosphereMaterial(anomalies=anomalies)
+    source = g.GaussianCurrent(latitude_deg=S.SOURCE_LAT, longitude_deg=S.SOURCE_LON,
+                               altitude_m=0.0, carrier_frequency_hz=20.0,
+                               one_over_e_half_width_s=1.5e-4, center_time_s=4.5e-4)
+    sim = g.GeodesicFDTD(cfg, material=material, source=source, backend="hip", dtype="float64")
+    assert sim.sigma_er.std(axis=0).max() > 0  # horizontally varying rows
+    ref = o.OracleSim(subdivision=2, radial_cells=12, minimum_altitude_m=-20e3,
+                      maximum_altitude_m=100e3, earth_radius_m=cfg.earth_radius_m,
+                      material_arrays=(sim.sigma_er, sim.epsilon_r_er, sim.sigma_et,
+                                       sim.epsilon_r_et),
+                      source=o.OracleSource(latitude_deg=S.SOURCE_LAT,
+                                            longitude_deg=S.SOURCE_LON, altitude_m=0.0,
+                                            carrier_frequency_hz=20.0,
+                                            one_over_e_half_width_s=1.5e-4,
+                                            center_time_s=4.5e-4),
+                      mesh=sim.mesh)
+    assert sim.time_step_s == ref.time_step_s
+    _seed_fields(sim, ref)
+    v, l, w = S.receiver_tables(sim.mesh, sim.altitudes_m, S.receiver_points(S.scenario("c0")))
+    traces = sim.record_er_observations(v, l, w, 150)
+    want = ref.record_er_observations(v, l, w, 150)
+    _compare(sim, ref, FP64_BAR, "cell-averaged materials fp64")
+    assert S.rel_l2(traces, want) <= FP64_BAR
