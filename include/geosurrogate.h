@@ -55,7 +55,10 @@ int geo_silu_forward(int64_t n, const float* pre, uint16_t* out_bf16, void* stre
 int geo_silu_backward(int64_t n, const float* pre, const float* dout, float* dpre,
                       uint16_t* dpre_bf16, void* stream);
 
-/* out[c] = sum over rows r < M of x[r*ld + c], c < N (deterministic order). */
+/* out[c] = sum over rows r < M of x[r*ld + c], c < N: fixed chunks of about
+ * 256 rows summed per block, then the chunk sums in order, so the result is
+ * the same bits on every call.  Takes a stream-ordered allocation for the
+ * chunk sums. */
 int geo_column_sums(int64_t M, int64_t N, const float* x, int64_t ld, float* out, void* stream);
 
 /* Nearest-neighbour upsampling by `up` followed by the im2col of a

@@ -206,20 +206,49 @@ __global__ void silu_backward_kernel(int64_t n, const float* __restrict__ pre,
   }
 }
 
-// One block per group of 64 columns; thread (row lane r, column c) sums rows
-// r, r + 4, ... and the 4 row lanes are combined in a fixed order.
-__global__ void __launch_bounds__(256) column_sums_kernel(int64_t M, int64_t N,
-                                                          const float* __restrict__ x, int64_t ld,
-                                                          float* __restrict__ out) {
-  __shared__ float part[4][64];
-  const int c = threadIdx.x & 63, r = threadIdx.x >> 6;
-  const int64_t col = (int64_t)blockIdx.x * 64 + c;
-  float s = 0.f;
-  if (col < N)
-    for (int64_t m = r; m < M; m += 4) s += x[m * ld + col];
-  part[r][c] = s;
-  __syncthreads();
-  if (r == 0 && col < N) out[col] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+// Column sums in two deterministic stages.  Stage 1: block b sums rows
+// [b * chunk, (b + 1) * chunk) of every column.  Its 256 threads cover
+// `width` = min(N, 256) adjacent columns (coalesced row segments) times
+// 256 / width row lanes; lane r takes rows r, r + lanes, ... and the lanes
+// of a column are added in lane order.  Stage 2 adds the block partials of
+// each column in block order.  The chunking depends on M only, so a sum is
+// the same bits on every call.
+constexpr int kColSumMaxBlocks = 1024;
+
+__global__ void __launch_bounds__(256) column_partials_kernel(int64_t M, int64_t N,
+                                                              const float* __restrict__ x,
+                                                              int64_t ld, int64_t chunk,
+                                                              float* __restrict__ partial) {
+  __shared__ float red[256];
+  const int t = threadIdx.x;
+  const int64_t lo = (int64_t)blockIdx.x * chunk;
+  const int64_t hi = lo + chunk < M ? lo + chunk : M;
+  const int width = N >= 256 ? 256 : (int)N;
+  const int lanes = 256 / width;
+  const int r = t / width, c = t - r * width;
+  for (int64_t c0 = 0; c0 < N; c0 += width) {
+    const int64_t col = c0 + c;
+    float sum = 0.f;
+    if (r < lanes && col < N)
+      for (int64_t m = lo + r; m < hi; m += lanes) sum += x[m * ld + col];
+    red[t] = sum;
+    __syncthreads();
+    if (t < width && col < N) {
+      float acc = red[t];
+      for (int q = 1; q < lanes; ++q) acc += red[q * width + t];
+      partial[(int64_t)blockIdx.x * N + col] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void column_reduce_kernel(int64_t N, int32_t blocks, const float* __restrict__ partial,
+                                     float* __restrict__ out) {
+  const int64_t col = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= N) return;
+  float acc = partial[col];
+  for (int32_t b = 1; b < blocks; ++b) acc += partial[(int64_t)b * N + col];
+  out[col] = acc;
 }
 
 __global__ void im2col_up_kernel(int64_t batch, int64_t length, int64_t channels, int32_t up,
@@ -452,9 +481,20 @@ int geo_silu_backward(int64_t n, const float* pre, const float* dout, float* dpr
 int geo_column_sums(int64_t M, int64_t N, const float* x, int64_t ld, float* out, void* stream) {
   if (M < 1 || N < 1 || ld < N || !x || !out)
     return geo_internal_fail(GEO_ERR_INVALID, "column sums: bad sizes");
-  hipLaunchKernelGGL(column_sums_kernel, dim3((uint32_t)((N + 63) / 64)), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), M, N, x, ld, out);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  // about 256 rows per block, at most kColSumMaxBlocks blocks
+  int64_t blocks = (M + 255) / 256;
+  if (blocks > kColSumMaxBlocks) blocks = kColSumMaxBlocks;
+  const int64_t chunk = (M + blocks - 1) / blocks;
+  blocks = (M + chunk - 1) / chunk;
+  float* partial = nullptr;
+  SG_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(float) * blocks * N, s));
+  hipLaunchKernelGGL(column_partials_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, M, N, x, ld,
+                     chunk, partial);
+  hipLaunchKernelGGL(column_reduce_kernel, dim3((uint32_t)((N + 255) / 256)), dim3(256), 0, s, N,
+                     (int32_t)blocks, (const float*)partial, out);
   SG_HIP(hipGetLastError());
+  SG_HIP(hipFreeAsync(partial, s));
   return GEO_OK;
 }
 

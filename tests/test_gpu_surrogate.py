@@ -114,6 +114,20 @@ def test_layout_reduction_and_optimizer_kernels():
                                       cs.data_ptr(), stream), lib)
     np.testing.assert_allclose(cs.cpu().numpy(), m.astype(np.float64).sum(axis=0), rtol=1e-5,
                                atol=1e-4)
+    # the training shapes (262 144 x 16 bias gradients), one column, more than
+    # 256 columns, a row pitch above N; the same bits on a second call
+    for rows, cols, ld in ((262144, 16, 16), (300001, 1, 1), (1000, 300, 300), (5, 3, 7)):
+        m = rng.standard_normal((rows, ld)).astype(np.float32)
+        md_ = torch.as_tensor(m, device="cuda")
+        got = []
+        for _ in range(2):
+            cs = torch.empty(cols, dtype=torch.float32, device="cuda")
+            _native.check(lib.geo_column_sums(rows, cols, md_.data_ptr(), ld, cs.data_ptr(),
+                                              stream), lib)
+            got.append(cs.cpu().numpy())
+        want = m[:, :cols].astype(np.float64).sum(axis=0)
+        np.testing.assert_allclose(got[0], want, rtol=0, atol=2e-6 * np.sqrt(rows) * 4)
+        assert got[0].tobytes() == got[1].tobytes()
     # Adam: one step from zero moments moves every parameter by -lr * sign(g)
     p = rng.standard_normal(500).astype(np.float32)
     g = rng.standard_normal(500).astype(np.float32)
