@@ -300,6 +300,70 @@ def ensemble_bench(args) -> dict:
     }
 
 
+# (label, M, N, K, A layout, B layout): "k" = K index contiguous, "r" = row
+# (M for A, N for B) contiguous
+GEMM_SHAPES = (
+    ("trunk layer, batch 2048", 2048, 256, 256, "k", "r"),
+    ("decoder convolution as im2col", 131072, 32, 160, "k", "r"),
+    ("decoder data gradient", 131072, 160, 32, "k", "k"),
+    ("weight gradient, split K", 256, 256, 131072, "r", "r"),
+    ("square 4096", 4096, 4096, 4096, "k", "r"),
+    ("square 8192", 8192, 8192, 8192, "k", "r"),
+)
+
+
+def gemm_bench(args) -> dict:
+    """geo_gemm_bf16 (BF16 in, FP32 out) beside torch's matmul (hipBLASLt, BF16
+    out) on the surrogate's own products and on large squares: HIP events on
+    the current stream, median of 20 launches after 3 warm-up launches."""
+    import numpy as np
+    import torch
+    from ionosphere_fdtd_amd import _native
+
+    lib = _native.load_library()
+    stream = torch.cuda.current_stream()
+    out = []
+    for label, M, N, K, al, bl in GEMM_SHAPES:
+        g = torch.Generator(device="cuda").manual_seed(0)
+        a = torch.randn((M, K) if al == "k" else (K, M), device="cuda", generator=g).bfloat16()
+        b = torch.randn((N, K) if bl == "k" else (K, N), device="cuda", generator=g).bfloat16()
+        a_rs, a_cs = (K, 1) if al == "k" else (1, M)
+        b_rs, b_cs = (1, K) if bl == "k" else (N, 1)
+        c = torch.empty((M, N), device="cuda", dtype=torch.float32)
+        at = a if al == "k" else a.t()
+        bt = b.t() if bl == "k" else b
+
+        def ours():
+            _native.check(lib.geo_gemm_bf16(M, N, K, a.data_ptr(), a_rs, a_cs, b.data_ptr(),
+                                            b_rs, b_cs, None, c.data_ptr(), None, N, 0,
+                                            stream.cuda_stream), lib)
+
+        def theirs():
+            torch.matmul(at, bt)
+
+        row = {"shape": label, "M": M, "N": N, "K": K}
+        for name, fn in (("geo_gemm_bf16", ours), ("hipblaslt", theirs)):
+            for _ in range(3):
+                fn()
+            times = []
+            for _ in range(20):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                fn()
+                e1.record(stream)
+                e1.synchronize()
+                times.append(e0.elapsed_time(e1))
+            ms = float(np.median(times))
+            row[name + "_tflops"] = round(2.0 * M * N * K / (ms * 1e-3) / 1e12, 1)
+            row[name + "_ms"] = round(ms, 4)
+        ref = (at.float() @ bt.float())
+        row["max_rel_err_vs_fp32"] = float((c - ref).abs().max() / ref.abs().max())
+        out.append(row)
+        print(json.dumps(row), file=sys.stderr, flush=True)
+    return {"metric": "BF16 GEMM TFLOP/s (FP32 accumulation)", "peak_dense_bf16_tflops": 2500.0,
+            "results": out}
+
+
 def surrogate_bench(args) -> dict:
     """Learned surrogate with FDTD fallback on held-out random-geometry queries.
 
@@ -423,7 +487,8 @@ def main() -> int:
                          "vacuum workload, beside the published rates")
     ap.add_argument("--workload", default="c1",
                     help="c1 (default), c1_noplasma, c1_pec, c2 (64-run ensemble), c3, "
-                         "c4 (surrogate + FDTD fallback), vacuum_s7_r64, vacuum_s7_r80")
+                         "c4 (surrogate + FDTD fallback), vacuum_s7_r64, vacuum_s7_r80, "
+                         "gemm (the surrogate's BF16 GEMM beside hipBLASLt)")
     ap.add_argument("--stream-ceiling", action="store_true",
                     help="also time a 1 GiB device copy (read + write GB/s)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -466,6 +531,9 @@ def main() -> int:
         return 0
     if args.workload == "c4":
         print(json.dumps(surrogate_bench(args)))
+        return 0
+    if args.workload == "gemm":
+        print(json.dumps(gemm_bench(args)))
         return 0
     if args.precision_sweep:
         sweep = precision_sweep(args)

@@ -48,93 +48,143 @@ __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint
 __device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + __expf(-x)); }
 
 // ---------------------------------------------------------------------------
-// GEMM.  Block tile 64 (M) x 64 (N), K step 32, 256 threads = 4 waves; wave w
-// owns the 32 x 32 quadrant (w / 2, w % 2) as 2 x 2 tiles of
-// v_mfma_f32_16x16x32_bf16.  Operand layout of that instruction: lane l holds
-// A[l % 16][8 (l / 16) + e] and B[8 (l / 16) + e][l % 16], e = 0..7, and
-// returns D[4 (l / 16) + r][l % 16], r = 0..3.  LDS holds A as [m][k] and B
-// transposed as [n][k], so every lane's operand is one 16-byte LDS read; rows
-// are padded by 8 values against bank conflicts.
+// GEMM.  256 threads = 4 waves in a 2 x 2 grid over a TM x TN block tile
+// (64 x 64 or 128 x 128), K step 32.  A wave owns a (TM/2) x (TN/2) quadrant
+// as (TM/32) x (TN/32) tiles of v_mfma_f32_16x16x32_bf16.  Operand layout of
+// that instruction: lane l holds A[l % 16][8 (l / 16) + e] and
+// B[8 (l / 16) + e][l % 16], e = 0..7, and returns D[4 (l / 16) + r][l % 16],
+// r = 0..3.  LDS holds A as [m][k] and B transposed as [n][k], so every
+// lane's operand is one 16-byte LDS read; rows are padded by 8 values against
+// bank conflicts.  Every output tile accumulates the same 32-wide K slices
+// in K order with the same instruction whatever the block tile, so the two
+// tile sizes give the same bits.
+//
+// Operand tiles are fetched into registers one K step ahead and written to
+// the other LDS buffer after the current step's MFMAs (double buffering, one
+// barrier per step).  A tile is fetched with 16-byte loads when its K index
+// is contiguous (8 consecutive k of one row per load) or its row index is
+// contiguous (8 consecutive rows at one k, scattered into the [row][k] LDS
+// layout); other strides, and the edges of the matrix, load element by
+// element with zero fill.
 // ---------------------------------------------------------------------------
-constexpr int kTM = 64, kTN = 64, kTK = 32, kLdsRow = kTK + 8;
+constexpr int kTK = 32, kLdsRow = kTK + 8;
+constexpr int kTM = 64, kTN = 64;  // tile of the split-K path (scratch sizing)
+enum : int { kOpGeneric = 0, kOpKContig = 1, kOpRowContig = 2 };
+
+// ROWS x kTK values of one operand, 8 per register vector, ROWS / 64 vectors
+// per thread.  X(row, k) = X[row * rs + k * ks].
+template <int ROWS, int MODE>
+struct OperandTile {
+  uint4 v[ROWS / 64];
+
+  __device__ __forceinline__ void fetch(const uint16_t* __restrict__ X, int64_t rs, int64_t ks,
+                                        int64_t row0, int64_t rows, int64_t k0, int64_t k_end) {
+#pragma unroll
+    for (int j = 0; j < ROWS / 64; ++j) {
+      const int q = (int)threadIdx.x + 256 * j;
+      // kOpRowContig: 8 consecutive rows at one k; otherwise 8 consecutive k of one row
+      const int kk = MODE == kOpRowContig ? q / (ROWS / 8) : (q & 3) * 8;
+      const int rr = MODE == kOpRowContig ? (q % (ROWS / 8)) * 8 : q >> 2;
+      const int64_t row = row0 + rr, k = k0 + kk;
+      if (MODE == kOpKContig && row < rows && k + 8 <= k_end) {
+        v[j] = *reinterpret_cast<const uint4*>(X + row * rs + k);
+      } else if (MODE == kOpRowContig && k < k_end && row + 8 <= rows) {
+        v[j] = *reinterpret_cast<const uint4*>(X + row + k * ks);
+      } else {
+        uint16_t e8[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int64_t r = MODE == kOpRowContig ? row + e : row;
+          const int64_t kx = MODE == kOpRowContig ? k : k + e;
+          e8[e] = (r < rows && kx < k_end) ? X[r * rs + kx * ks] : (uint16_t)0;
+        }
+        v[j].x = (uint32_t)e8[0] | ((uint32_t)e8[1] << 16);
+        v[j].y = (uint32_t)e8[2] | ((uint32_t)e8[3] << 16);
+        v[j].z = (uint32_t)e8[4] | ((uint32_t)e8[5] << 16);
+        v[j].w = (uint32_t)e8[6] | ((uint32_t)e8[7] << 16);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(uint16_t* lds) const {
+#pragma unroll
+    for (int j = 0; j < ROWS / 64; ++j) {
+      const int q = (int)threadIdx.x + 256 * j;
+      if (MODE == kOpRowContig) {
+        const int kk = q / (ROWS / 8), rr = (q % (ROWS / 8)) * 8;
+        const uint32_t w[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          lds[(rr + e) * kLdsRow + kk] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+      } else {
+        *reinterpret_cast<uint4*>(&lds[(q >> 2) * kLdsRow + (q & 3) * 8]) = v[j];
+      }
+    }
+  }
+};
 
 // Split K: block z covers k in [z * k_chunk, (z + 1) * k_chunk) and, with
 // more than one split, writes its raw partial product to partial[z][m][n];
 // gemm_reduce_kernel then adds the partials in split order (deterministic).
+template <int TM, int TN, int AMODE, int BMODE>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(
     int64_t M, int64_t N, int64_t K, const uint16_t* __restrict__ A, int64_t a_rs, int64_t a_cs,
     const uint16_t* __restrict__ B, int64_t b_rs, int64_t b_cs, const float* __restrict__ bias,
     float* __restrict__ c32, uint16_t* __restrict__ c16, int64_t ldc, int32_t accumulate,
     int64_t k_chunk, float* __restrict__ partial) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[kTM * kLdsRow];
-  __shared__ __attribute__((aligned(16))) uint16_t Bt[kTN * kLdsRow];
+  constexpr int FI = TM / 32, FJ = TN / 32;  // 16 x 16 tiles per wave along m, n
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][TM * kLdsRow];
+  __shared__ __attribute__((aligned(16))) uint16_t Bt[2][TN * kLdsRow];
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int64_t m0 = (int64_t)blockIdx.y * kTM, n0 = (int64_t)blockIdx.x * kTN;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
-  f32x4 acc[2][2];
+  const int64_t m0 = (int64_t)blockIdx.y * TM, n0 = (int64_t)blockIdx.x * TN;
+  const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
+  f32x4 acc[FI][FJ];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-  // loader coordinates: tile row (0..63) and 8 consecutive k
-  const int lr = t >> 2, lk = (t & 3) * 8;
+    for (int j = 0; j < FJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
   const int64_t k_begin = (int64_t)blockIdx.z * k_chunk;
   const int64_t k_end = k_begin + k_chunk < K ? k_begin + k_chunk : K;
+  OperandTile<TM, AMODE> ta;
+  OperandTile<TN, BMODE> tb;
+  ta.fetch(A, a_rs, a_cs, m0, M, k_begin, k_end);
+  tb.fetch(B, b_cs, b_rs, n0, N, k_begin, k_end);
+  ta.store(As[0]);
+  tb.store(Bt[0]);
+  __syncthreads();
+  int buf = 0;
   for (int64_t k0 = k_begin; k0 < k_end; k0 += kTK) {
-    {
-      const int64_t m = m0 + lr;
-      uint16_t v[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int64_t k = k0 + lk + e;
-        v[e] = (m < M && k < k_end) ? A[m * a_rs + k * a_cs] : (uint16_t)0;
-      }
-      uint4 packed;
-      packed.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
-      packed.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
-      packed.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
-      packed.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
-      *reinterpret_cast<uint4*>(&As[lr * kLdsRow + lk]) = packed;
+    const bool more = k0 + kTK < k_end;
+    if (more) {  // next step's operands in flight during this step's MFMAs
+      ta.fetch(A, a_rs, a_cs, m0, M, k0 + kTK, k_end);
+      tb.fetch(B, b_cs, b_rs, n0, N, k0 + kTK, k_end);
     }
-    {
-      const int64_t n = n0 + lr;
-      uint16_t v[8];
+    bf16x8 a[FI], b[FJ];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        const int64_t k = k0 + lk + e;
-        v[e] = (n < N && k < k_end) ? B[k * b_rs + n * b_cs] : (uint16_t)0;
-      }
-      uint4 packed;
-      packed.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
-      packed.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
-      packed.z = (uint32_t)v[4] | ((uint32_t)v[5] << 16);
-      packed.w = (uint32_t)v[6] | ((uint32_t)v[7] << 16);
-      *reinterpret_cast<uint4*>(&Bt[lr * kLdsRow + lk]) = packed;
-    }
-    __syncthreads();
-    bf16x8 a[2], b[2];
+    for (int i = 0; i < FI; ++i)
+      a[i] = __builtin_bit_cast(
+          bf16x8, *reinterpret_cast<const uint4*>(&As[buf][(wm + i * 16 + r16) * kLdsRow + kq]));
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const uint4 x = *reinterpret_cast<const uint4*>(&As[(wm + i * 16 + r16) * kLdsRow + kq]);
-      a[i] = __builtin_bit_cast(bf16x8, x);
-    }
+    for (int j = 0; j < FJ; ++j)
+      b[j] = __builtin_bit_cast(
+          bf16x8, *reinterpret_cast<const uint4*>(&Bt[buf][(wn + j * 16 + r16) * kLdsRow + kq]));
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const uint4 x = *reinterpret_cast<const uint4*>(&Bt[(wn + j * 16 + r16) * kLdsRow + kq]);
-      b[j] = __builtin_bit_cast(bf16x8, x);
-    }
+    for (int i = 0; i < FI; ++i)
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < FJ; ++j)
         acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+    if (more) {  // the other buffer: every wave finished reading it before the last barrier
+      ta.store(As[buf ^ 1]);
+      tb.store(Bt[buf ^ 1]);
+    }
     __syncthreads();
+    buf ^= 1;
   }
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < FI; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int j = 0; j < FJ; ++j)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t m = m0 + wm + i * 16 + (lane >> 4) * 4 + r;
@@ -376,6 +426,44 @@ int gemm_scratch_address(float** out) {
   return GEO_OK;
 }
 
+// Load mode of an operand X(row, k) = X[row * rs + k * ks]: 16-byte vectors
+// along whichever index is contiguous, when the base and the other stride
+// keep every vector 16-byte aligned (rows and K steps start at multiples of 8).
+int operand_mode(const uint16_t* X, int64_t rs, int64_t ks) {
+  const bool aligned = (reinterpret_cast<uintptr_t>(X) & 15u) == 0;
+  if (aligned && ks == 1 && rs % 8 == 0) return kOpKContig;
+  if (aligned && rs == 1 && ks % 8 == 0) return kOpRowContig;
+  return kOpGeneric;
+}
+
+template <int TM, int TN, int AM, int BM>
+void launch_gemm_kernel(dim3 grid, hipStream_t s, int64_t M, int64_t N, int64_t K,
+                        const uint16_t* A, int64_t a_rs, int64_t a_cs, const uint16_t* B,
+                        int64_t b_rs, int64_t b_cs, const float* bias, float* c32, uint16_t* c16,
+                        int64_t ldc, int32_t acc, int64_t k_chunk, float* partial) {
+  hipLaunchKernelGGL((gemm_bf16_kernel<TM, TN, AM, BM>), grid, dim3(256), 0, s, M, N, K, A, a_rs,
+                     a_cs, B, b_rs, b_cs, bias, c32, c16, ldc, acc, k_chunk, partial);
+}
+
+template <int TM, int TN>
+void launch_gemm(int am, int bm, dim3 grid, hipStream_t s, int64_t M, int64_t N, int64_t K,
+                 const uint16_t* A, int64_t a_rs, int64_t a_cs, const uint16_t* B, int64_t b_rs,
+                 int64_t b_cs, const float* bias, float* c32, uint16_t* c16, int64_t ldc,
+                 int32_t acc, int64_t k_chunk, float* partial) {
+#define GEMM_CASE(a_, b_)                                                                    \
+  case a_ * 3 + b_:                                                                          \
+    launch_gemm_kernel<TM, TN, a_, b_>(grid, s, M, N, K, A, a_rs, a_cs, B, b_rs, b_cs, bias, \
+                                       c32, c16, ldc, acc, k_chunk, partial);                \
+    break;
+  switch (am * 3 + bm) {
+    GEMM_CASE(0, 0) GEMM_CASE(0, 1) GEMM_CASE(0, 2)
+    GEMM_CASE(1, 0) GEMM_CASE(1, 1) GEMM_CASE(1, 2)
+    GEMM_CASE(2, 0) GEMM_CASE(2, 1) GEMM_CASE(2, 2)
+    default: break;
+  }
+#undef GEMM_CASE
+}
+
 // C = A B (+ bias).  `scratch_partial` NULL: split-K partials come from the
 // stream-ordered allocator; otherwise they go to that buffer of
 // kGemmScratchFloats values (device address of `scratch`).
@@ -402,10 +490,19 @@ int gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs, 
   }
   const int64_t k_chunk = ((K + splits - 1) / splits + kTK - 1) / kTK * kTK;
   splits = (K + k_chunk - 1) / k_chunk;
+  const int am = operand_mode(A, a_rs, a_cs), bm = operand_mode(B, b_cs, b_rs);
   if (splits == 1) {
-    hipLaunchKernelGGL(gemm_bf16_kernel, dim3((uint32_t)gx, (uint32_t)gy, 1), dim3(256), 0, s, M,
-                       N, K, A, a_rs, a_cs, B, b_rs, b_cs, bias, c_f32, c_bf16, ldc, accumulate,
-                       K, (float*)nullptr);
+    // 128 x 128 block tiles where both sides fill them, 64 x 64 otherwise
+    if (M >= 128 && N >= 128) {
+      const int64_t gy2 = (M + 127) / 128, gx2 = (N + 127) / 128;
+      launch_gemm<128, 128>(am, bm, dim3((uint32_t)gx2, (uint32_t)gy2, 1), s, M, N, K, A, a_rs,
+                            a_cs, B, b_rs, b_cs, bias, c_f32, c_bf16, ldc, accumulate, K,
+                            (float*)nullptr);
+    } else {
+      launch_gemm<kTM, kTN>(am, bm, dim3((uint32_t)gx, (uint32_t)gy, 1), s, M, N, K, A, a_rs,
+                            a_cs, B, b_rs, b_cs, bias, c_f32, c_bf16, ldc, accumulate, K,
+                            (float*)nullptr);
+    }
     SG_HIP(hipGetLastError());
     return GEO_OK;
   }
@@ -417,9 +514,9 @@ int gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs, 
   } else {
     SG_HIP(hipMallocAsync(reinterpret_cast<void**>(&partial), sizeof(float) * splits * M * N, s));
   }
-  hipLaunchKernelGGL(gemm_bf16_kernel, dim3((uint32_t)gx, (uint32_t)gy, (uint32_t)splits),
-                     dim3(256), 0, s, M, N, K, A, a_rs, a_cs, B, b_rs, b_cs, (const float*)nullptr,
-                     (float*)nullptr, (uint16_t*)nullptr, ldc, 0, k_chunk, partial);
+  launch_gemm<kTM, kTN>(am, bm, dim3((uint32_t)gx, (uint32_t)gy, (uint32_t)splits), s, M, N, K,
+                        A, a_rs, a_cs, B, b_rs, b_cs, (const float*)nullptr, (float*)nullptr,
+                        (uint16_t*)nullptr, ldc, 0, k_chunk, partial);
   hipLaunchKernelGGL(gemm_reduce_kernel, dim3(grid_for(M * N)), dim3(256), 0, s, M, N,
                      (int32_t)splits, (const float*)partial, bias, c_f32, c_bf16, ldc, accumulate);
   SG_HIP(hipGetLastError());

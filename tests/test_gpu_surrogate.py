@@ -68,6 +68,60 @@ def test_gemm_bf16_matches_fp64_product_of_bf16_operands(M, N, K, ta, tb):
     assert np.all(np.abs(got16 - want) <= 2.0 ** -8 * np.abs(want) + 2e-5 * scale + 1e-5)
 
 
+@pytest.mark.parametrize("M,N,K", [(200, 136, 77), (129, 300, 64), (1000, 130, 40)])
+def test_gemm_tile_paths_and_operand_layouts_give_the_same_bits(M, N, K):
+    """Every operand layout (K contiguous, row contiguous, a stride of 2 that
+    takes the element-wise loads) and both block tiles: products with M, N >=
+    128 run on 128 x 128 tiles, the same rows in chunks of 64 on 64 x 64
+    tiles; each output tile accumulates the same K slices in the same order,
+    so all must agree bit for bit, and with the FP64 product of the BF16
+    operands to FP32 accumulation error."""
+    import torch
+
+    from ionosphere_fdtd_amd import _native
+    from ionosphere_fdtd_amd.surrogate import bf16_bits, bf16_to_f32
+
+    lib = _native.load_library()
+    stream = torch.cuda.current_stream().cuda_stream
+    rng = np.random.default_rng(M + 7 * N + 13 * K)
+    a16 = bf16_bits(rng.standard_normal((M, K)).astype(np.float32))
+    b16 = bf16_bits(rng.standard_normal((K, N)).astype(np.float32))
+    want = bf16_to_f32(a16).astype(np.float64) @ bf16_to_f32(b16).astype(np.float64)
+
+    def operand(x16, layout):
+        # (device buffer, row stride, column stride) of the (rows, cols) matrix x
+        rows, cols = x16.shape
+        if layout == "row-major":
+            return _dev16(torch, x16), cols, 1
+        if layout == "col-major":
+            return _dev16(torch, x16.T.copy()), 1, rows
+        wide = np.zeros((rows, 2 * cols), dtype=np.uint16)  # every other column
+        wide[:, ::2] = x16
+        return _dev16(torch, wide), 2 * cols, 2
+
+    results = []
+    for la in ("row-major", "col-major", "strided"):
+        for lb in ("row-major", "col-major", "strided"):
+            A, a_rs, a_cs = operand(a16, la)
+            B, b_rs, b_cs = operand(b16, lb)
+            full = torch.empty((M, N), dtype=torch.float32, device="cuda")
+            _native.check(lib.geo_gemm_bf16(M, N, K, A.data_ptr(), a_rs, a_cs, B.data_ptr(), b_rs,
+                                            b_cs, None, full.data_ptr(), None, N, 0, stream), lib)
+            chunks = torch.empty((M, N), dtype=torch.float32, device="cuda")
+            for m0 in range(0, M, 64):
+                rows = min(64, M - m0)
+                off = (m0 * a_rs) * 2  # bytes
+                _native.check(lib.geo_gemm_bf16(rows, N, K, A.data_ptr() + off, a_rs, a_cs,
+                                                B.data_ptr(), b_rs, b_cs, None,
+                                                chunks[m0:].data_ptr(), None, N, 0, stream), lib)
+            got = full.cpu().numpy()
+            assert got.tobytes() == chunks.cpu().numpy().tobytes(), (la, lb)
+            np.testing.assert_allclose(got, want, rtol=0, atol=2e-5 * np.sqrt(K) + 1e-5)
+            results.append(got)
+    for r in results[1:]:
+        assert r.tobytes() == results[0].tobytes()
+
+
 def test_layout_reduction_and_optimizer_kernels():
     import torch
 
