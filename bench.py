@@ -308,6 +308,7 @@ GEMM_SHAPES = (
     ("decoder data gradient", 131072, 160, 32, "k", "k"),
     ("weight gradient, split K", 256, 256, 131072, "r", "r"),
     ("square 4096", 4096, 4096, 4096, "k", "r"),
+    ("square 4096, B stored transposed", 4096, 4096, 4096, "k", "k"),
     ("square 8192", 8192, 8192, 8192, "k", "r"),
 )
 
@@ -501,7 +502,7 @@ def main() -> int:
     ap.add_argument("--queries", type=int, default=2000, help="held-out queries (--workload c4)")
     ap.add_argument("--receivers-per-config", type=int, default=1,
                     help="queries sharing one configuration (--workload c4)")
-    ap.add_argument("--disagreement", default="0.015,0.01",
+    ap.add_argument("--disagreement", default="0.01,0.015,0.02",
                     help="committee disagreement above which a query goes to FDTD "
                          "(--workload c4); a comma-separated list times the service at "
                          "each, the first one reported as the result")

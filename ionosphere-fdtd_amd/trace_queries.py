@@ -701,7 +701,7 @@ class TraceQueryService:
     """
 
     def __init__(self, model: SurrogateCommittee, mesh=None, *, tolerance: float = 0.02,
-                 disagreement: float = 0.015, spot_checks: int = 1, seed: int = 0):
+                 disagreement: float = 0.01, spot_checks: int = 1, seed: int = 0):
         if isinstance(model, SurrogateModel):
             model = SurrogateCommittee([model])
         self.model = model
