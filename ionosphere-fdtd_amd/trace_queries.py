@@ -161,6 +161,66 @@ class MeshRings:
         self.mean_edge = float(self.edge_length.mean())
         self.unit = np.asarray(mesh.vertices, dtype=np.float64)
         self._hoods: dict[int, tuple[np.ndarray, np.ndarray]] = {}
+        # padded adjacency (-1 past a vertex's degree) and sorted edge keys
+        # min * V + max, for the vectorised pair lookups
+        self.adjacent6 = np.full((V, 6), -1, dtype=np.int64)
+        for v in range(V):
+            self.adjacent6[v, :counts[v]] = self.adjacent[v]
+        ekey = np.minimum(e[:, 0], e[:, 1]) * V + np.maximum(e[:, 0], e[:, 1])
+        self._edge_order = np.argsort(ekey, kind="stable")
+        self._edge_keys = ekey[self._edge_order]
+        self._pair_keys = None
+        self._pair_rings = None
+
+    def _ring_table(self):
+        """Sorted keys u * V + v of every ordered pair within `rings` rings and
+        their ring numbers: breadth-first layers of the sparse adjacency."""
+        if self._pair_keys is None:
+            from scipy import sparse
+
+            V = self.mesh.n_vertices
+            e = np.asarray(self.mesh.edges, dtype=np.int64)
+            adj = sparse.coo_matrix((np.ones(2 * len(e), dtype=np.int8),
+                                     (np.concatenate((e[:, 0], e[:, 1])),
+                                      np.concatenate((e[:, 1], e[:, 0])))), shape=(V, V)).tocsr()
+            seen = sparse.identity(V, dtype=np.int8, format="csr")
+            keys = [np.arange(V, dtype=np.int64) * (V + 1)]
+            rings = [np.zeros(V, dtype=np.int64)]
+            frontier = seen
+            for r in range(1, self.rings + 1):
+                reach = ((frontier @ adj) > 0).astype(np.int8)
+                new = (reach - reach.multiply(seen)).tocoo()
+                new.eliminate_zeros()
+                keys.append(new.row.astype(np.int64) * V + new.col.astype(np.int64))
+                rings.append(np.full(new.nnz, r, dtype=np.int64))
+                seen = ((seen + new.tocsr()) > 0).astype(np.int8)
+                frontier = new.tocsr()
+            k = np.concatenate(keys)
+            order = np.argsort(k)
+            self._pair_keys, self._pair_rings = k[order], np.concatenate(rings)[order]
+        return self._pair_keys, self._pair_rings
+
+    def rings_of(self, u, v) -> np.ndarray:
+        """Vectorised ring(u, v): ring numbers of the pairs, rings + 1 beyond."""
+        keys, rings = self._ring_table()
+        q = np.asarray(u, dtype=np.int64) * self.mesh.n_vertices + np.asarray(v, dtype=np.int64)
+        at = np.minimum(np.searchsorted(keys, q), len(keys) - 1)
+        return np.where(keys[at] == q, rings[at], self.rings + 1)
+
+    def common_neighbours_of(self, u, v) -> np.ndarray:
+        """Vectorised common_neighbours(u, v)."""
+        a = self.adjacent6[np.asarray(u, dtype=np.int64)]
+        b = self.adjacent6[np.asarray(v, dtype=np.int64)]
+        return np.sum((a[:, :, None] == b[:, None, :]) & (a[:, :, None] >= 0), axis=(1, 2))
+
+    def pair_couplings(self, u, v) -> np.ndarray:
+        """Vectorised pair_coupling(u, v) for adjacent pairs (NaN otherwise)."""
+        u = np.asarray(u, dtype=np.int64)
+        v = np.asarray(v, dtype=np.int64)
+        q = np.minimum(u, v) * self.mesh.n_vertices + np.maximum(u, v)
+        at = np.minimum(np.searchsorted(self._edge_keys, q), len(self._edge_keys) - 1)
+        hit = self._edge_keys[at] == q
+        return np.where(hit, self.edge_coupling[self._edge_order[at]], np.nan)
 
     def neighbourhood(self, u: int) -> tuple[np.ndarray, np.ndarray]:
         """Vertices within `rings` rings of u (u first) and their ring numbers."""
@@ -216,6 +276,24 @@ def drude_profile(h_prime_km, beta_per_km, midpoint_altitudes_m):
     return np.where(z >= 60.0, density, 0.0), 1.8e11 * np.exp(-0.15 * z)
 
 
+def time_steps(h_prime_km, beta_per_km, mesh) -> np.ndarray:
+    """time_step for arrays of (h', beta), one value per run (the same
+    elementwise operations, evaluated for all runs at once)."""
+    h = np.atleast_1d(np.asarray(h_prime_km, dtype=np.float64))[:, None]
+    b = np.atleast_1d(np.asarray(beta_per_km, dtype=np.float64))[:, None]
+    alt = _altitudes()
+    radii = EARTH_RADIUS_M + alt
+    mids = 0.5 * (radii[1:] + radii[:-1]) - EARTH_RADIUS_M
+    smallest = float(radii.min())
+    inv = (smallest * mesh.primal_edge_angles.min()) ** -2 + \
+        (smallest * mesh.dual_edge_angles.min()) ** -2 + (2.0 / np.diff(radii).min()) ** 2
+    geometric = 1.0 / (C_0 * np.sqrt(inv))
+    density, _ = drude_profile(h, b, mids[None, :])
+    wp = np.sqrt(density.max(axis=1) * ELECTRON_CHARGE_C ** 2 / ELECTRON_MASS_KG / EPSILON_0)
+    with np.errstate(divide="ignore"):
+        return 0.35 * np.where(wp > 0, np.minimum(geometric, 2.0 / wp), geometric)
+
+
 def time_step(h_prime_km, beta_per_km, mesh) -> float:
     """The run's time step: courant 0.35 x min(geometric CFL, 2 / omega_p max)
     (the solver's rule, solver.py:377-406, for a vacuum-permittivity grid)."""
@@ -250,7 +328,7 @@ def pair_features(physics: np.ndarray, dts: np.ndarray, u: np.ndarray, v: np.nda
     x[:, 4] = np.log(np.asarray(dts) / 1e-7)
     dist = np.arccos(np.clip(np.sum(rings.unit[u] * rings.unit[v], axis=1), -1.0, 1.0)) \
         * EARTH_RADIUS_M
-    ring = np.asarray([rings.ring(a, b) for a, b in zip(u, v)])
+    ring = rings.rings_of(u, v)
     x[:, 5] = dist / 100e3
     x[:, 6] = ring == 1
     x[:, 7] = ring == 2
@@ -260,9 +338,10 @@ def pair_features(physics: np.ndarray, dts: np.ndarray, u: np.ndarray, v: np.nda
     x[:, 11] = (rings.edge_length[u] / rings.mean_edge - 1.0) * 10.0
     x[:, 12] = (rings.edge_length[v] / rings.mean_edge - 1.0) * 10.0
     x[:, 13] = (rings.degree[u] == 5).astype(float) + (rings.degree[v] == 5)
-    x[:, 14] = [((rings.pair_coupling(a, b) / rings.mean_coupling - 1.0) * 10.0) if r == 1
-                else (rings.common_neighbours(a, b) / 2.0 if r == 2 else 0.0)
-                for a, b, r in zip(u, v, ring)]
+    with np.errstate(invalid="ignore"):
+        x[:, 14] = np.where(ring == 1, (rings.pair_couplings(u, v) / rings.mean_coupling - 1.0)
+                            * 10.0,
+                            np.where(ring == 2, rings.common_neighbours_of(u, v) / 2.0, 0.0))
     x[:, 15] = np.log1p(dist / (C_0 * np.asarray(dts)) / STEPS)
     return x.astype(np.float32)
 
@@ -503,23 +582,25 @@ def decompose(queries: np.ndarray, mesh):
     dist = np.arccos(np.clip(np.sum(src * rx, axis=1), -1.0, 1.0)) * EARTH_RADIUS_M
     covered &= dist <= MAX_DISTANCE_M * (1.0 + 1e-9)
     _prefetch_tables(mesh, np.concatenate((q[:, 3:5], q[:, 5:7])))
-    for i, row in enumerate(q):
-        su, _, sw = _receiver_table(mesh, alt, row[3], row[4])
-        rv, _, rw = _receiver_table(mesh, alt, row[5], row[6])
-        for u, a in zip(su, sw):
-            if a == 0.0:
-                continue
-            for v, b in zip(rv, rw):
-                if b == 0.0:
-                    continue
-                if rings.ring(int(u), int(v)) > rings.rings:
-                    covered[i] = False
-                qi.append(i)
-                us.append(int(u))
-                vs.append(int(v))
-                ws.append(a * b)
-    return (np.asarray(qi, dtype=np.int64), np.asarray(us, dtype=np.int64),
-            np.asarray(vs, dtype=np.int64), np.asarray(ws)), covered
+    n = q.shape[0]
+    src = [_receiver_table(mesh, alt, row[3], row[4]) for row in q]
+    rcv = [_receiver_table(mesh, alt, row[5], row[6]) for row in q]
+    ks = max((len(t[0]) for t in src), default=1)
+    kr = max((len(t[0]) for t in rcv), default=1)
+    su, sw = np.zeros((n, ks), dtype=np.int64), np.zeros((n, ks))
+    rv, rw = np.zeros((n, kr), dtype=np.int64), np.zeros((n, kr))
+    for i in range(n):  # padding entries carry weight 0 and drop out below
+        su[i, :len(src[i][0])], sw[i, :len(src[i][2])] = src[i][0], src[i][2]
+        rv[i, :len(rcv[i][0])], rw[i, :len(rcv[i][2])] = rcv[i][0], rcv[i][2]
+    # pairs in the order (query, source entry, receiver entry), zero weights dropped
+    keep = ((sw != 0.0)[:, :, None] & (rw != 0.0)[:, None, :]).ravel()
+    qi = np.repeat(np.arange(n, dtype=np.int64), ks * kr)[keep]
+    us = np.broadcast_to(su[:, :, None], (n, ks, kr)).ravel()[keep]
+    vs = np.broadcast_to(rv[:, None, :], (n, ks, kr)).ravel()[keep]
+    ws = (sw[:, :, None] * rw[:, None, :]).ravel()[keep]
+    far = rings.rings_of(us, vs) > rings.rings
+    covered[np.unique(qi[far])] = False
+    return (qi, us, vs, ws), covered
 
 
 def in_range(queries: np.ndarray, mesh=None) -> np.ndarray:
@@ -638,7 +719,7 @@ class SurrogateCommittee:
         trace from the committee mean."""
         q = np.asarray(queries, dtype=np.float64)
         (qi, u, v, w), _ = decompose(q, mesh)
-        dts = np.asarray([time_step(h, b, mesh) for h, b in q[:, :2]])
+        dts = time_steps(q[:, 0], q[:, 1], mesh)
         feats = pair_features(q[qi][:, [0, 1, 2, 7]], dts[qi], u, v, mesh_rings(mesh))
         per_member = []
         for m in self.members:
@@ -719,6 +800,9 @@ class TraceQueryService:
         ok = in_range(q, self.mesh)
         report = {"queries": int(q.shape[0]), "in_range": int(ok.sum())}
         fdtd_rows = np.flatnonzero(~ok)
+        rows = np.zeros(0, dtype=np.int64)
+        check = np.zeros(0, dtype=bool)
+        pred = None
         if ok.any():
             rows = np.flatnonzero(ok)
             t1 = time.perf_counter()
@@ -735,23 +819,27 @@ class TraceQueryService:
                                         replace=False)
                 chosen = {distinct[p] for p in picks}
                 check = np.asarray([k in chosen for k in keys])
-                t2 = time.perf_counter()
-                truth = fdtd_traces(q[rows[check]], self.mesh, batch=64)
-                report["spot_check_seconds"] = time.perf_counter() - t2
-                errors = trace_rel_l2(pred[check], truth)
-                report["spot_check_errors"] = errors.tolist()
-                if np.all(errors <= self.tolerance):
-                    out[rows] = pred
-                    out[rows[check]] = truth
-                    served[rows] = "surrogate"
-                    served[rows[check]] = "fdtd"
-                else:
-                    fdtd_rows = np.concatenate((fdtd_rows, rows))
-                    report["fallback"] = "spot check failed"
-        if fdtd_rows.size:
+        # One FDTD pass for the out-of-range and uncertain queries and the
+        # spot-checked configurations (one set of ensembles instead of two).
+        first = np.concatenate((fdtd_rows, rows[check])).astype(np.int64)
+        if first.size:
             t3 = time.perf_counter()
-            out[fdtd_rows] = fdtd_traces(q[fdtd_rows], self.mesh)
-            report["fallback_fdtd_seconds"] = time.perf_counter() - t3
+            out[first] = fdtd_traces(q[first], self.mesh)
+            report["fdtd_seconds"] = time.perf_counter() - t3
+        if rows.size:
+            errors = trace_rel_l2(pred[check], out[rows[check]])
+            report["spot_check_errors"] = errors.tolist()
+            if np.all(errors <= self.tolerance):
+                out[rows[~check]] = pred[~check]
+                served[rows[~check]] = "surrogate"
+            else:
+                rest = rows[~check]
+                report["fallback"] = "spot check failed"
+                if rest.size:
+                    t4 = time.perf_counter()
+                    out[rest] = fdtd_traces(q[rest], self.mesh)
+                    report["fdtd_seconds"] = report.get("fdtd_seconds", 0.0) + \
+                        time.perf_counter() - t4
         report["seconds"] = time.perf_counter() - t0
         report["served_by_surrogate"] = int(np.sum(served == "surrogate"))
         return out, served, report
