@@ -46,8 +46,10 @@
  * calling thread's last failure.
  *
  * Threading: one host thread per plan.  All device work is enqueued on the
- * caller's stream; no call synchronises the host except geo_diagnostics and
- * geo_radial_profiles.
+ * caller's stream; no call synchronises the host except geo_diagnostics,
+ * geo_radial_profiles, and building a step graph (geo_step_prepare, or the
+ * first geo_step of a chunk size), which waits for the graph's upload to the
+ * device.
  */
 #ifndef GEOFDTD_H
 #define GEOFDTD_H

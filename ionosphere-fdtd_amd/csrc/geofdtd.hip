@@ -1283,6 +1283,10 @@ struct Plan final : PlanBase {
     GEO_HIP(hipStreamEndCapture(capture_stream, &g));
     graph = g;
     GEO_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    // upload the executable graph now, so that its first replay does not
+    // carry the one-off upload (the replays themselves are unchanged)
+    GEO_HIP(hipGraphUpload(graph_exec, capture_stream));
+    GEO_HIP(hipStreamSynchronize(capture_stream));
     graph_chunk = chunk;
     return GEO_OK;
   }
@@ -1839,6 +1843,10 @@ struct Ensemble final : EnsembleBase {
     GEO_HIP(hipStreamEndCapture(capture_stream, &g));
     graph = g;
     GEO_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+    // upload the executable graph now, so that its first replay does not
+    // carry the one-off upload (the replays themselves are unchanged)
+    GEO_HIP(hipGraphUpload(graph_exec, capture_stream));
+    GEO_HIP(hipStreamSynchronize(capture_stream));
     graph_chunk = chunk;
     return GEO_OK;
   }
