@@ -72,6 +72,10 @@ extern "C" {
 #define GEO_STORAGE_F16 2
 #define GEO_STORAGE_BF16 3
 
+#define GEO_MAPPING_AUTO 0
+#define GEO_MAPPING_ROWS 1
+#define GEO_MAPPING_FLAT 2
+
 #define GEO_MAX_SOURCE 16
 #define GEO_MAX_SI_TERMS 32
 #define GEO_MAX_SPECIES 4
@@ -168,7 +172,13 @@ typedef struct GeoPhysicsDesc {
    * low-discrepancy sequence).  Runs are repeatable bit for bit.  Ignored
    * for F32/F64. */
   int32_t storage_rounding;
-  int32_t reserved4;
+  /* Thread mapping of the two step kernels (GEO_MAPPING_*): the strips of
+   * each block either belong to whole rows (an integer number of rows per
+   * 256-thread block) or run through the rows flat (rows may straddle
+   * blocks, every lane busy).  Both give the same bits; AUTO takes FLAT for
+   * a kernel whose whole-row blocks would leave at least 16 lanes idle in
+   * one of its segments. */
+  int32_t thread_mapping;
 } GeoPhysicsDesc;
 
 /* Device buffers owned by the caller.  Pitches count elements. */
@@ -199,6 +209,8 @@ typedef struct GeoLayout {
   int64_t radial_closed_form; /* 1 = uniform radii and FP32 arithmetic: the kernels form
                                  1/r and the radial-derivative weights from r_0 and dr
                                  instead of reading per-layer tables */
+  int64_t h_thread_mapping; /* GEO_MAPPING_ROWS or GEO_MAPPING_FLAT taken by the H kernel */
+  int64_t e_thread_mapping; /* ... and by the E kernel */
 } GeoLayout;
 
 typedef struct GeoDiagOut {

@@ -67,7 +67,7 @@ class GeoPhysicsDesc(C.Structure):
         ("plasma_coef", _p_f64), ("plasma_coef_rows", C.c_int64),
         ("plasma_b", _p_f64), ("plasma_b_rows", C.c_int64),
         ("sigma_er", _p_f64), ("sigma_et", _p_f64),
-        ("storage_rounding", C.c_int32), ("reserved4", C.c_int32),
+        ("storage_rounding", C.c_int32), ("thread_mapping", C.c_int32),
     ]
 
 
@@ -85,7 +85,8 @@ class GeoLayout(C.Structure):
                 ("plasma_components", C.c_int64),
                 ("vector_elems", C.c_int64), ("compute_bytes", C.c_int64),
                 ("si_pitch", C.c_int64), ("plasma_basis", C.c_int64),
-                ("radial_closed_form", C.c_int64)]
+                ("radial_closed_form", C.c_int64), ("h_thread_mapping", C.c_int64),
+                ("e_thread_mapping", C.c_int64)]
 
 
 class GeoDiagOut(C.Structure):

@@ -12,6 +12,7 @@
 #include <cfloat>
 #include <cmath>
 #include <cstddef>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -119,7 +120,8 @@ std::vector<uint32_t> build_block_map(const std::vector<SegSpec>& segs) {
 }
 
 // Rows of `pitch` elements in 16-byte strips of `strip_elems`, S strips per
-// thread; an integer number of rows per 256-thread block.
+// thread; an integer number of rows per 256-thread block (the flat mapping
+// is switched on afterwards by set_flat).
 geo::SegGeom make_geom(int64_t rows, int64_t pitch, int64_t strip_elems, int32_t S = 1) {
   geo::SegGeom g;
   g.n_rows = (int32_t)rows;
@@ -127,10 +129,38 @@ geo::SegGeom make_geom(int64_t rows, int64_t pitch, int64_t strip_elems, int32_t
   g.strips = (g.row_strips + S - 1) / S;
   g.rows_per_block = geo::kBlock / g.strips;
   g.magic = (uint32_t)((65536 + g.strips - 1) / g.strips);
+  g.flat = 0;
   return g;
 }
 
+// Lanes a whole-row block leaves without a strip.
+int32_t idle_lanes(const geo::SegGeom& g) { return geo::kBlock - g.rows_per_block * g.strips; }
+
+// Whether a kernel whose segments are `a` and `b` runs flat: the mapping
+// asked for, or under AUTO when whole-row blocks would leave a sixteenth of
+// their lanes idle in either segment.  Measured on the MI355X (ms per
+// launch, whole-row -> flat): at 96 radial cells (rows of 25 and 24
+// strips, 6 and 16 idle lanes) E 0.311 -> 0.235 and H 0.299 -> 0.288 at
+// s7, E 1.235 -> 0.933 at s8; at 80 cells (4 and 16 idle) E 0.169 ->
+// 0.139; at 64 cells (1 and 0 idle) flat is slower: E 0.148 -> 0.150,
+// H 0.184 -> 0.189.  Flat indexing needs rows * strips < 2^31.
+bool use_flat(int32_t mapping, const geo::SegGeom& a, const geo::SegGeom& b) {
+  const auto fits = [](const geo::SegGeom& g) {
+    return (int64_t)g.n_rows * g.row_strips + geo::kBlock < (int64_t)INT32_MAX;
+  };
+  if (!fits(a) || !fits(b) || mapping == GEO_MAPPING_ROWS) return false;
+  if (mapping == GEO_MAPPING_FLAT) return true;
+  return idle_lanes(a) >= geo::kBlock / 16 || idle_lanes(b) >= geo::kBlock / 16;
+}
+
+void set_flat(geo::SegGeom* g) {
+  g->flat = 1;
+  g->strips = g->row_strips;
+}
+
 int32_t geom_blocks(const geo::SegGeom& g) {
+  if (g.flat)
+    return (int32_t)(((int64_t)g.n_rows * g.row_strips + geo::kBlock - 1) / geo::kBlock);
   return (int32_t)((g.n_rows + g.rows_per_block - 1) / g.rows_per_block);
 }
 
@@ -527,6 +557,7 @@ struct Plan final : PlanBase {
   geo::HParams<T, C> hp{};
   geo::EParams<T, C> ep{};
   int32_t h_grid = 0, e_grid = 0;
+  bool h_flat = false, e_flat = false;  // thread mapping of each kernel
   geo::Dynamic* dyn = nullptr;
   bool bound = false;
   int32_t rounding_mode = 0;  // narrow stores: 0 nearest, 1 / 2 stochastic thresholds
@@ -716,6 +747,8 @@ struct Plan final : PlanBase {
     if (ph->scaling != 0 && ph->scaling != 1)
       return fail(GEO_ERR_INVALID, "storage scaling must be 0 (static) or 1 (dynamic)");
     dyn_scaling = narrow && ph->scaling == 1;
+    if (ph->thread_mapping < GEO_MAPPING_AUTO || ph->thread_mapping > GEO_MAPPING_FLAT)
+      return fail(GEO_ERR_INVALID, "thread mapping must be 0 (auto), 1 (rows) or 2 (flat)");
     if (ph->storage_rounding < 0 || ph->storage_rounding > 2)
       return fail(GEO_ERR_INVALID,
                   "storage rounding must be 0 (nearest), 1 (independent) or 2 (rotation)");
@@ -969,6 +1002,11 @@ struct Plan final : PlanBase {
     hp.n_edges = (int32_t)E;
     hp.g_edge = make_geom(E, pn, N);
     hp.g_face = make_geom(F, pm, N);
+    h_flat = use_flat(ph->thread_mapping, hp.g_edge, hp.g_face);
+    if (h_flat) {
+      set_flat(&hp.g_edge);
+      set_flat(&hp.g_face);
+    }
     {
       std::vector<SegSpec> segs = {{geo::kSegHEdge, geom_blocks(hp.g_edge)},
                                    {geo::kSegHFace, geom_blocks(hp.g_face)}};
@@ -1028,6 +1066,11 @@ struct Plan final : PlanBase {
     ep.dyn = dyn;
     ep.g_vertex = make_geom(V, pn, N);
     ep.g_edge = make_geom(E, pm, N);
+    e_flat = use_flat(ph->thread_mapping, ep.g_vertex, ep.g_edge);
+    if (e_flat) {
+      set_flat(&ep.g_vertex);
+      set_flat(&ep.g_edge);
+    }
     {
       std::vector<SegSpec> segs = {{geo::kSegEVertex, geom_blocks(ep.g_vertex)},
                                    {geo::kSegEEdge, geom_blocks(ep.g_edge)}};
@@ -1114,6 +1157,8 @@ struct Plan final : PlanBase {
     layout.plasma_components = ncomp;
     layout.plasma_basis = plasma && !magnetized ? 1 : 0;
     layout.radial_closed_form = hp.rg.uniform;
+    layout.h_thread_mapping = h_flat ? GEO_MAPPING_FLAT : GEO_MAPPING_ROWS;
+    layout.e_thread_mapping = e_flat ? GEO_MAPPING_FLAT : GEO_MAPPING_ROWS;
     layout.vector_elems = N;
     layout.compute_bytes = sizeof(C);
     layout.si_pitch = si_on ? hp.si_pitch : 0;

@@ -39,15 +39,19 @@ enum Segment : uint32_t {
   kSegEVertex = 4, kSegEEdge = 5, kSegHSample = 6,
 };
 
-// Column segment geometry: an integer number of columns per 256-thread
-// block, column index within the block by a 16-bit reciprocal multiply
-// (exact for tid < 256 and strips <= 256).
+// Column segment geometry.  Whole-row mapping: an integer number of rows
+// per 256-thread block, column index within the block by a 16-bit
+// reciprocal multiply (exact for tid < 256 and strips <= 256).  Flat
+// mapping (one strip per thread only): thread t of block b takes strip
+// b * 256 + t of the segment's row-major strip sequence, so rows may
+// straddle blocks and every lane has work.
 struct SegGeom {
   int32_t n_rows;
   int32_t strips;         // thread columns per row: ceil(row_strips / S)
   int32_t rows_per_block;
   uint32_t magic;         // ceil(65536 / strips)
   int32_t row_strips;     // 16-byte strips per row
+  int32_t flat;           // 1: flat mapping (strips == row_strips)
 };
 
 struct Dynamic {
@@ -377,6 +381,15 @@ __device__ __forceinline__ int32_t logical_block(int32_t per_xcd) {
 __device__ __forceinline__ void row_and_strip(const SegGeom& g, int32_t block, int32_t* row,
                                               int32_t* strip) {
   const uint32_t tid = threadIdx.x;
+  if (g.flat) {
+    // rows * row_strips < 2^31 (checked on the host); rows past the end
+    // come out >= n_rows and the caller skips them
+    const uint32_t s = (uint32_t)block * (uint32_t)kBlock + tid;
+    const uint32_t r = s / (uint32_t)g.row_strips;
+    *row = (int32_t)r;
+    *strip = (int32_t)(s - r * (uint32_t)g.row_strips);
+    return;
+  }
   const uint32_t local_row = (tid * g.magic) >> 16;
   *strip = (int32_t)(tid - local_row * (uint32_t)g.strips);
   *row = (local_row < (uint32_t)g.rows_per_block)

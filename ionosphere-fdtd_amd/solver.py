@@ -44,6 +44,8 @@ _FIELD_ORDER = ("er", "et", "hr", "ht")
 
 # Rounding of narrow (FP16/BF16) stored values: the C ABI's storage_rounding codes.
 STORAGE_ROUNDING = {"nearest": 0, "stochastic": 1, "rotation": 2}
+# Thread mapping of the step kernels: the C ABI's GEO_MAPPING_* codes.
+THREAD_MAPPING = {"auto": 0, "rows": 1, "flat": 2}
 # A source envelope longer than this many steps drives quasi-static fields.
 STAGNATION_STEPS = 1.0e4
 
@@ -189,6 +191,11 @@ class GeodesicFDTD:
     stagnates once a field's per-step change drops below half an ulp, as it
     does under slowly varying sources (the 20 Hz benchmark source: O(1)
     errors after 5000-10000 steps); the constructor warns for that case.
+    ``thread_mapping`` picks how the step kernels' threads cover the field
+    rows: ``"rows"`` (whole rows per 256-thread block), ``"flat"`` (rows may
+    straddle blocks, every lane busy) or ``"auto"`` (default: flat for a
+    kernel whose whole-row blocks would leave 16 or more lanes idle, as at
+    80 or 96 radial cells).  All three give the same bits.
     """
 
     def __init__(
@@ -208,6 +215,7 @@ class GeodesicFDTD:
         storage_scales: dict[str, float] | None = None,
         storage_scaling: str = "auto",
         storage_rounding: str = "rotation",
+        thread_mapping: str = "auto",
     ) -> None:
         if (isinstance(compile_chunk_size, bool)
                 or not isinstance(compile_chunk_size, (int, np.integer))
@@ -216,6 +224,9 @@ class GeodesicFDTD:
         if storage_rounding not in STORAGE_ROUNDING:
             raise ValueError("storage_rounding must be one of " + ", ".join(
                 repr(k) for k in STORAGE_ROUNDING))
+        if thread_mapping not in THREAD_MAPPING:
+            raise ValueError("thread_mapping must be one of " + ", ".join(
+                repr(k) for k in THREAD_MAPPING))
         self.compile_chunk_size = int(compile_chunk_size)
         self.config = config or SimulationConfig()
         cfg = self.config
@@ -313,6 +324,7 @@ class GeodesicFDTD:
         # to nearest, every update smaller than half an ulp of the stored
         # value is lost, which freezes slowly varying (quasi-static) fields.
         self.storage_rounding = storage_rounding
+        self.thread_mapping = thread_mapping
         self._initial_scales = scales
         self.time_s = 0.0
         self.steps = 0
@@ -424,6 +436,7 @@ class GeodesicFDTD:
             sigma_er=f64(sig_er), sigma_et=f64(sig_et),
             scaling=int(narrow and self.storage_scaling == "dynamic"),
             storage_rounding=STORAGE_ROUNDING[self.storage_rounding] if narrow else 0,
+            thread_mapping=THREAD_MAPPING[self.thread_mapping],
             scale_er=self._initial_scales["er"] if narrow else 1.0,
             scale_et=self._initial_scales["et"] if narrow else 1.0,
             scale_hr=self._initial_scales["hr"] if narrow else 1.0,

@@ -258,7 +258,7 @@ def _plasma_arrays(sc, mesh, midpoint_altitudes):
 
 def build_sim(sc: dict, *, compile_step=False, compile_chunk_size=8, dtype=None,
               storage_scales=None, storage_scaling="auto", mesh=None,
-              storage_rounding="rotation"):
+              storage_rounding="rotation", thread_mapping="auto"):
     import ionosphere_fdtd_amd as g
 
     cfg = g.SimulationConfig(
@@ -282,7 +282,8 @@ def build_sim(sc: dict, *, compile_step=False, compile_chunk_size=8, dtype=None,
     kwargs = dict(material=material, surface_impedance=surface, plasma=plasma, mesh=mesh,
                   backend="hip", dtype=dtype or sc["dtype"], compile_step=compile_step,
                   compile_chunk_size=compile_chunk_size, storage_scales=storage_scales,
-                  storage_scaling=storage_scaling, storage_rounding=storage_rounding)
+                  storage_scaling=storage_scaling, storage_rounding=storage_rounding,
+                  thread_mapping=thread_mapping)
     from ionosphere_fdtd_amd.solver import stable_time_step
 
     timed = [v for part in (sc["source"] or {}, sc["tangential"] or {}) for v in part.values()

@@ -319,6 +319,8 @@ def test_nearest_narrow_storage_with_slow_source_warns(monkeypatch):
     with pytest.raises(ValueError, match="storage_rounding"):
         g.GeodesicFDTD(g.SimulationConfig(subdivision=0, radial_cells=2), dtype="float16",
                        storage_rounding="up")
+    with pytest.raises(ValueError, match="thread_mapping"):
+        g.GeodesicFDTD(g.SimulationConfig(subdivision=0, radial_cells=2), thread_mapping="cols")
 
 
 def test_trace_query_rings_and_pair_weights():
