@@ -27,6 +27,7 @@ sys.path.insert(0, str(REPO))
 sys.path.insert(0, str(REPO / "tests"))
 
 HBM_PEAK_GBS = 8000.0
+MAPPING_NAMES = {1: "rows", 2: "flat"}
 C2_RUN_STEPS = 4000  # length of one ensemble run (16 receiver traces per run)
 METRIC = "Mcell-updates/s + % HBM roofline (1 GPU); ensemble trace queries/s at ≤2% err"
 
@@ -72,10 +73,11 @@ def graph_chunk(steps: int, limit: int) -> int:
     return 1
 
 
-def read_traffic(kernel: str) -> float | None:
+def read_traffic(kernel: str, workload: str) -> float | None:
     """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE,
-    scaled by the copy-kernel calibration), from the committed summary."""
-    path = REPO / "profiles" / f"{kernel}_c1_traffic.txt"
+    scaled by the copy-kernel calibration), from the committed summary of that
+    workload (None when there is none)."""
+    path = REPO / "profiles" / f"{kernel}_{workload}_traffic.txt"
     if not path.exists():
         return None
     for line in path.read_text().splitlines():
@@ -551,7 +553,8 @@ def main() -> int:
     pts = S.receiver_points(sc)
     v, l, w = S.receiver_tables(sim.mesh, sim.altitudes_m, pts)
     setup_s = time.perf_counter() - t_setup
-    print(f"setup {setup_s:.1f} s  V={sim.mesh.n_vertices} R=64 dt={sim.time_step_s:.4e}",
+    print(f"setup {setup_s:.1f} s  V={sim.mesh.n_vertices} R={len(sim.radial_midpoints_m)} "
+          f"dt={sim.time_step_s:.4e}",
           file=sys.stderr, flush=True)
     # Receivers, the trace buffer, the per-step source currents and the step
     # graph are prepared before the clock starts: the timed region is exactly
@@ -612,12 +615,16 @@ def main() -> int:
                    "time_step_s": sim.time_step_s, "graph_chunk": chunk,
                    "timed_path": (f"{args.steps // chunk} HIP graph replays of {chunk} steps"
                                   if chunk > 1 else f"{args.steps} single-step launches"),
+                   "thread_mapping": {"h_step_kernel": MAPPING_NAMES[sim._layout.h_thread_mapping],
+                                      "e_step_kernel": MAPPING_NAMES[sim._layout.e_thread_mapping]},
                    "event_ms_per_step": round(event_ms / args.steps, 5)},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic(dominant),
+                     "traffic": read_traffic(dominant, args.workload),
                      "algorithmic_bytes": dom_bytes,
+                     "algorithmic_bytes_per_kernel": {"h_step_kernel": bytes_["h"],
+                                                      "e_step_kernel": bytes_["e"]},
                      "kernel_ms": {"h_step_kernel": round(h_ms, 5), "e_step_kernel": round(e_ms, 5)},
                      "step_fraction": round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
     }
