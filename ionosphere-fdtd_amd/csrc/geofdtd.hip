@@ -99,6 +99,15 @@ struct SegSpec {
   int32_t blocks;
 };
 
+// A block-map entry holds the block index within its segment in 24 bits.
+constexpr int32_t kMaxSegmentBlocks = 1 << 24;
+
+bool blocks_fit(const std::vector<SegSpec>& segs) {
+  for (const SegSpec& sp : segs)
+    if (sp.blocks < 0 || sp.blocks >= kMaxSegmentBlocks) return false;
+  return true;
+}
+
 // Interleave the blocks of several segments in proportion to their sizes so
 // that, e.g., edge and face blocks covering the same mesh region run close
 // together; entry = segment << 24 | block index within the segment.
@@ -1011,6 +1020,8 @@ struct Plan final : PlanBase {
       std::vector<SegSpec> segs = {{geo::kSegHEdge, geom_blocks(hp.g_edge)},
                                    {geo::kSegHFace, geom_blocks(hp.g_face)}};
       segs.push_back({geo::kSegSample, 1});
+      if (!blocks_fit(segs))
+        return fail(GEO_ERR_INVALID, "mesh too large: a kernel segment needs 2^24 blocks or more");
       std::vector<uint32_t> map = build_block_map(segs);
       uint32_t* d_map;
       if ((rc = arena.upload(map, &d_map)) != GEO_OK) return rc;
@@ -1075,6 +1086,8 @@ struct Plan final : PlanBase {
       std::vector<SegSpec> segs = {{geo::kSegEVertex, geom_blocks(ep.g_vertex)},
                                    {geo::kSegEEdge, geom_blocks(ep.g_edge)}};
       segs.push_back({geo::kSegHSample, 1});  // H traces; returns at once without them
+      if (!blocks_fit(segs))
+        return fail(GEO_ERR_INVALID, "mesh too large: a kernel segment needs 2^24 blocks or more");
       std::vector<uint32_t> map = build_block_map(segs);
       uint32_t* d_map;
       if ((rc = arena.upload(map, &d_map)) != GEO_OK) return rc;
@@ -1829,9 +1842,11 @@ struct Ensemble final : EnsembleBase {
       std::vector<SegSpec> hs = {{geo::kSegHEdge, geom_blocks(host_hp[r].g_edge)},
                                  {geo::kSegHFace, geom_blocks(host_hp[r].g_face)}};
       hs.push_back({geo::kSegSample, 1});
+      const std::vector<SegSpec> es = {{geo::kSegEVertex, geom_blocks(host_ep[r].g_vertex)},
+                                       {geo::kSegEEdge, geom_blocks(host_ep[r].g_edge)}};
+      if (!blocks_fit(hs) || !blocks_fit(es)) return fail(GEO_ERR_INVALID, "run too large for an ensemble");
       hm[r] = build_block_map(hs);
-      em[r] = build_block_map({{geo::kSegEVertex, geom_blocks(host_ep[r].g_vertex)},
-                               {geo::kSegEEdge, geom_blocks(host_ep[r].g_edge)}});
+      em[r] = build_block_map(es);
       if ((rc = encode(hm[r], (uint32_t)r)) != GEO_OK) return rc;
       if ((rc = encode(em[r], (uint32_t)r)) != GEO_OK) return rc;
     }
