@@ -981,8 +981,18 @@ __device__ __forceinline__ void h_dispatch(const HParams<T, C>& p, const Scales<
   }
 }
 
+// Narrow storage (8-value strips) leaves the compiler at 95 VGPRs and 5
+// waves per SIMD for the H kernel with Drude plasma; asking for 6 waves
+// fits it in 80 VGPRs without spills (s8 x 96 FP16: 1.021 -> 0.980 ms,
+// s7 x 64: 0.177 -> 0.171 ms).  The same request spills the magnetised
+// variant, and the E kernel (108 VGPRs), so neither gets it.
+template <typename T, bool MAGNETIZED>
+constexpr int h_min_waves() { return (sizeof(T) == 2 && !MAGNETIZED) ? 6 : 1; }
+
 template <typename T, typename C, bool PLASMA, bool MAGNETIZED>
-__global__ void __launch_bounds__(kBlock) h_step_kernel(const HParams<T, C> p) {
+__global__ void __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(h_min_waves<T, MAGNETIZED>())))
+h_step_kernel(const HParams<T, C> p) {
   const int32_t L = logical_block(p.per_xcd);
   if (L >= p.n_blocks) return;
   const Scales<C> sc = load_scales<T, C>(p.dyn);
@@ -1200,24 +1210,32 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
   load_strip(hrow + j0, ht, sc.ht_inv);
   T* erow = p.et + (size_t)e * p.pm + j0;
   load_strip(erow, et, sc.et_inv);
+  // Loss coefficients and radial weights.  Narrow storage (8 values per
+  // strip) requests them only after the plasma reads: issued with the
+  // other loads they held the FP16 kernel at 108 VGPRs and 4 waves per SIMD
+  // against 92 and 5 (s7 x 64 FP16 E 0.122 -> 0.109 ms, s8 x 96 0.725 ->
+  // 0.680 ms).  They are a few radial profiles that stay in L2.
   C cl[N], cb[N], irb[N], ra[N];
-  load_vec<N>(p.cl_et + (size_t)e * p.ca_et_stride + j0, cl);
-  load_vec<N>(p.cb_et + (size_t)e * p.ca_et_stride + j0, cb);
-  if (sizeof(C) == 4 && p.rg.uniform) {
-    // alpha'_j = r_j / (dr rbar_j) = 1/dr - 1/(2 rbar_j) (spherical), 1/dr (flat)
+  auto load_tables = [&]() {
+    load_vec<N>(p.cl_et + (size_t)e * p.ca_et_stride + j0, cl);
+    load_vec<N>(p.cb_et + (size_t)e * p.ca_et_stride + j0, cb);
+    if (sizeof(C) == 4 && p.rg.uniform) {
+      // alpha'_j = r_j / (dr rbar_j) = 1/dr - 1/(2 rbar_j) (spherical), 1/dr (flat)
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-      const int32_t j = j0 + i;
-      const C inv = j < p.R ? inv_radius(p.rg, (C)j + (C)0.5) : (C)0;
-      irb[i] = inv;
-      ra[i] = j < p.R ? p.rg.inv_dr - (p.spherical ? (C)0.5 * inv : (C)0) : (C)0;
+      for (int i = 0; i < N; ++i) {
+        const int32_t j = j0 + i;
+        const C inv = j < p.R ? inv_radius(p.rg, (C)j + (C)0.5) : (C)0;
+        irb[i] = inv;
+        ra[i] = j < p.R ? p.rg.inv_dr - (p.spherical ? (C)0.5 * inv : (C)0) : (C)0;
+      }
+    } else {
+      load_vec<N>(p.inv_rbar + j0, irb);
+      load_vec<N>(p.rh_a + j0, ra);
     }
-  } else {
-    load_vec<N>(p.inv_rbar + j0, irb);
-    load_vec<N>(p.rh_a + j0, ra);
-  }
-  // every load (the plasma reads below included) is issued before the first
-  // wait, the lane shuffle after them
+  };
+  if constexpr (!is_narrow<T>()) load_tables();
+  // every load (the plasma reads below included; for narrow storage all but
+  // the tables) is issued before the first wait, the lane shuffle after them
   C ht_next = (C)0;
   if (!ln.right && j0 + N <= p.R) ht_next = load_one(hrow + j0 + N, sc.ht_inv);
   C jt[N];
@@ -1268,6 +1286,7 @@ __device__ __forceinline__ void e_edge_strip(const EParams<T, C>& p, const Scale
       for (int i = 0; i < N; ++i) jt[i] = (C)0.5 * (sl[i] + sr[i]);
     }
   }
+  if constexpr (is_narrow<T>()) load_tables();
   const C from_right = __shfl_down(ht[0], 1);
   ht[N] = ln.right ? from_right : ht_next;
   C src[N];
