@@ -47,9 +47,12 @@
  *
  * Threading: one host thread per plan.  All device work is enqueued on the
  * caller's stream; no call synchronises the host except geo_diagnostics,
- * geo_radial_profiles, and building a step graph (geo_step_prepare, or the
+ * geo_radial_profiles, building a step graph (geo_step_prepare, or the
  * first geo_step of a chunk size), which waits for the graph's upload to the
- * device.
+ * device, and geo_set_receivers / geo_set_h_receivers, which replace their
+ * device tables with hipFree, hipMalloc and blocking copies (call them
+ * between steps, not while work that reads the old tables is queued on
+ * another stream).
  */
 #ifndef GEOFDTD_H
 #define GEOFDTD_H
