@@ -1785,6 +1785,15 @@ struct Ensemble final : EnsembleBase {
       host_hp[r].g_face = make_geom(p.F, p.pm, N, S);
       host_ep[r].g_vertex = make_geom(p.V, p.pn, N, S);
       host_ep[r].g_edge = make_geom(p.E, p.pm, N, S);
+      // one strip per thread: each run keeps the thread mapping its plan took
+      if (S == 1 && p.h_flat) {
+        set_flat(&host_hp[r].g_edge);
+        set_flat(&host_hp[r].g_face);
+      }
+      if (S == 1 && p.e_flat) {
+        set_flat(&host_ep[r].g_vertex);
+        set_flat(&host_ep[r].g_edge);
+      }
       seen[r] = p.generation;
     }
   }

@@ -70,6 +70,29 @@ def test_ensemble_runs_equal_runs_alone_bitwise(strips, order):
     assert np.any(traces != 0)
 
 
+@pytest.mark.parametrize("strips", [1, 2])
+def test_ensemble_of_flat_mapped_runs_equals_runs_alone_bitwise(strips):
+    """Runs of 80 and 96 radial cells take the flat thread mapping; with one
+    strip per thread the ensemble keeps it, with two it runs whole rows."""
+    import ionosphere_fdtd_amd as g
+
+    scs = _scenarios(n=3, radial=(80, 96))
+    solo = _build(scs, "float32", compile_step=True, compile_chunk_size=20)
+    assert all(s._layout.e_thread_mapping == 2 for s in solo)
+    rx = _receivers(solo, scs)
+    solo_traces = [s.record_er_observations(*r, 60) for s, r in zip(solo, rx)]
+    runs = _build(scs, "float32")
+    ens = g.EnsembleFDTD(runs, compile_chunk_size=20, autotune=False)
+    ens.configure(strips, 0)
+    traces = ens.record_er_observations(rx, 60)
+    for i, (a, b) in enumerate(zip(runs, solo)):
+        np.testing.assert_array_equal(traces[i], solo_traces[i])
+        sa, sb = _state(a), _state(b)
+        for k in sa:
+            np.testing.assert_array_equal(sa[k], sb[k], err_msg=f"run {i} {k}")
+    assert np.any(traces != 0)
+
+
 def test_ensemble_fp64_matches_oracle():
     import ionosphere_fdtd_amd as g
 
