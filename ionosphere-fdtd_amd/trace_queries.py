@@ -770,7 +770,7 @@ def train(features: np.ndarray, traces: np.ndarray, *, steps: int = 6000, batch:
 
 
 class TraceQueryService:
-    """Answer trace queries with the surrogate committee, FDTD staying the judge.
+    """Answer trace queries with the surrogate committee, FDTD staying the arbiter of error.
 
     In-range queries get the committee's mean trace unless (a) its members
     disagree by more than `disagreement` -- that query goes to FDTD -- or (b)
