@@ -514,8 +514,8 @@ def main() -> int:
                          "nearest instead of stochastically)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: the multi-GPU halo path is out of scope for this "
-              "build; run with --gpus 1", file=sys.stderr)
+        print("bench.py measures one GPU: the multi-GPU halo exchange is not implemented; "
+              "run with --gpus 1", file=sys.stderr)
         return 2
     import numpy as np
     import torch
