@@ -1698,9 +1698,11 @@ __global__ void __launch_bounds__(geo::kBlock) pattern_probe_kernel(f4v* __restr
 // Column-block shape (the tuned parameters):
 //   S       strips per thread (1, 2 or 4): a thread updates S 16-byte strips
 //           of one row, amortising the row's index and metric loads;
-//   policy  block order: 0 run-major (each XCD sweeps whole runs, so a run's
-//           neighbour columns stay in one L2), 1 runs interleaved block by
-//           block (every XCD works on every run).
+//   policy  bit 0, block order: 0 run-major (each XCD sweeps whole runs, so
+//           a run's neighbour columns stay in one L2), 1 runs interleaved
+//           block by block (every XCD works on every run); bit 1 (one strip
+//           per thread only): every run takes the flat thread mapping, where
+//           otherwise each run keeps the mapping its plan took.
 // ---------------------------------------------------------------------------
 struct EnsembleBase {
   virtual ~EnsembleBase() = default;
@@ -1785,12 +1787,14 @@ struct Ensemble final : EnsembleBase {
       host_hp[r].g_face = make_geom(p.F, p.pm, N, S);
       host_ep[r].g_vertex = make_geom(p.V, p.pn, N, S);
       host_ep[r].g_edge = make_geom(p.E, p.pm, N, S);
-      // one strip per thread: each run keeps the thread mapping its plan took
-      if (S == 1 && p.h_flat) {
+      // one strip per thread: each run keeps the thread mapping its plan took,
+      // or every run goes flat (policy bit 1)
+      const bool all_flat = (policy & 2) != 0;
+      if (S == 1 && (p.h_flat || all_flat)) {
         set_flat(&host_hp[r].g_edge);
         set_flat(&host_hp[r].g_face);
       }
-      if (S == 1 && p.e_flat) {
+      if (S == 1 && (p.e_flat || all_flat)) {
         set_flat(&host_ep[r].g_vertex);
         set_flat(&host_ep[r].g_edge);
       }
@@ -1811,7 +1815,7 @@ struct Ensemble final : EnsembleBase {
   static std::vector<uint32_t> combine(const std::vector<std::vector<uint32_t>>& per_run,
                                        int32_t policy) {
     std::vector<uint32_t> out;
-    if (policy == 0) {
+    if ((policy & 1) == 0) {
       for (const auto& v : per_run) out.insert(out.end(), v.begin(), v.end());
       return out;
     }
@@ -1839,7 +1843,9 @@ struct Ensemble final : EnsembleBase {
 
   int configure(int32_t s, int32_t pol) override {
     if (!(s == 1 || s == 2 || s == 4)) return fail(GEO_ERR_INVALID, "strips per thread: 1, 2 or 4");
-    if (pol != 0 && pol != 1) return fail(GEO_ERR_INVALID, "block order policy: 0 or 1");
+    if (pol < 0 || pol > 3) return fail(GEO_ERR_INVALID, "block order policy: 0 to 3");
+    if ((pol & 2) && s != 1)
+      return fail(GEO_ERR_INVALID, "the flat-mapping policy takes one strip per thread");
     S = s;
     policy = pol;
     drop_graph();
@@ -1981,7 +1987,8 @@ struct Ensemble final : EnsembleBase {
   }
 
   // Time every (S, policy) candidate over `steps` source-free steps and keep
-  // the fastest; the state of every run is restored afterwards.
+  // the fastest (the flat-mapping policies only with S = 1); the state of
+  // every run is restored afterwards.
   int tune(int32_t steps, hipStream_t stream, int32_t* best_s, int32_t* best_policy,
            double* best_ms) override {
     if (steps < 1) return fail(GEO_ERR_INVALID, "tuning needs at least one step");
@@ -2002,8 +2009,9 @@ struct Ensemble final : EnsembleBase {
     GEO_HIP(hipEventCreate(&b));
     double best = 1e300;
     int32_t bs = 1, bp = 0;
-    for (int32_t pol : {0, 1}) {
+    for (int32_t pol : {0, 1, 2, 3}) {
       for (int32_t s : {1, 2, 4}) {
+        if ((pol & 2) && s != 1) continue;
         if ((rc = configure(s, pol)) != GEO_OK) break;
         if ((rc = step(nullptr, 2, 1, nullptr, stream)) != GEO_OK) break;  // warm-up
         GEO_HIP(hipEventRecord(a, stream));

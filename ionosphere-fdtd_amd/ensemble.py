@@ -80,6 +80,9 @@ class EnsembleFDTD:
         return self.runs[0]._stream()
 
     def configure(self, strips_per_thread: int, block_order: int) -> None:
+        """Column-block shape: 1, 2 or 4 strips per thread; block order 0
+        (run-major) or 1 (runs interleaved), plus 2 for every run on the flat
+        thread mapping (one strip per thread only)."""
         _native.check(self._lib.geo_ensemble_configure(self._handle, int(strips_per_thread),
                                                        int(block_order)), self._lib)
         self.shape = {"strips_per_thread": int(strips_per_thread),

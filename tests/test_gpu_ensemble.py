@@ -46,7 +46,8 @@ def _state(sim):
     return out
 
 
-@pytest.mark.parametrize("strips,order", [(1, 0), (2, 0), (4, 0), (1, 1), (2, 1), (4, 1)])
+@pytest.mark.parametrize("strips,order", [(1, 0), (2, 0), (4, 0), (1, 1), (2, 1), (4, 1),
+                                          (1, 2), (1, 3)])
 def test_ensemble_runs_equal_runs_alone_bitwise(strips, order):
     import ionosphere_fdtd_amd as g
 
@@ -125,7 +126,8 @@ def test_autotuner_restores_state_and_picks_a_shape():
     before = [_state(r) for r in runs]
     shape = ens.tune(5)
     print("tuned", shape, flush=True)
-    assert shape["strips_per_thread"] in (1, 2, 4) and shape["block_order"] in (0, 1)
+    assert shape["strips_per_thread"] in (1, 2, 4) and shape["block_order"] in (0, 1, 2, 3)
+    assert shape["strips_per_thread"] == 1 or shape["block_order"] < 2
     assert shape["ms_per_step"] > 0
     for b, r in zip(before, runs):
         a = _state(r)
@@ -156,6 +158,10 @@ def test_ensemble_rejects_mixed_runs():
     short = (rx[1][0][:2], rx[1][1][:2], rx[1][2][:2])
     with pytest.raises(ValueError, match="receivers"):
         ens.record_er_observations([rx[0], short], 5)
+    with pytest.raises(ValueError, match="one strip per thread"):
+        ens.configure(2, 2)
+    with pytest.raises(ValueError, match="policy"):
+        ens.configure(1, 4)
     # the same run twice: refused by the Python layer and by the C ABI itself
     with pytest.raises(ValueError, match="only once"):
         g.EnsembleFDTD([runs[0], runs[1], runs[0]], autotune=False)
