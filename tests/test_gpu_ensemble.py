@@ -71,15 +71,17 @@ def test_ensemble_runs_equal_runs_alone_bitwise(strips, order):
     assert np.any(traces != 0)
 
 
-@pytest.mark.parametrize("strips", [1, 2])
-def test_ensemble_of_flat_mapped_runs_equals_runs_alone_bitwise(strips):
-    """Runs of 80 and 96 radial cells take the flat thread mapping; with one
-    strip per thread the ensemble keeps it, with two it runs whole rows."""
+@pytest.mark.parametrize("strips,radial", [(1, (80, 96)), (2, (80, 96)), (1, (24, 96))])
+def test_ensemble_of_flat_mapped_runs_equals_runs_alone_bitwise(strips, radial):
+    """Runs of 80 and 96 radial cells take the flat thread mapping, runs of 24
+    whole rows; with one strip per thread the ensemble keeps each run's
+    mapping (mixed in one launch for 24 and 96), with two it runs whole rows."""
     import ionosphere_fdtd_amd as g
 
-    scs = _scenarios(n=3, radial=(80, 96))
+    scs = _scenarios(n=4, radial=radial)
     solo = _build(scs, "float32", compile_step=True, compile_chunk_size=20)
-    assert all(s._layout.e_thread_mapping == 2 for s in solo)
+    for sc, s in zip(scs, solo):
+        assert s._layout.e_thread_mapping == (2 if sc["radial_cells"] >= 80 else 1)
     rx = _receivers(solo, scs)
     solo_traces = [s.record_er_observations(*r, 60) for s, r in zip(solo, rx)]
     runs = _build(scs, "float32")
