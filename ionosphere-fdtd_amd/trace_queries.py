@@ -402,27 +402,42 @@ def surface_tables(mesh, lats, lons) -> list:
     planes = _face_planes(mesh)
     signs = np.stack([np.einsum("nkj,nj->nk", p[near], dirs) for p in planes], axis=2)
     inside = np.all(signs >= -1.0e-12, axis=2) | np.all(signs <= 1.0e-12, axis=2)
-    out = []
-    for i in range(len(dirs)):
-        cand = np.flatnonzero(inside[i])
-        if len(cand):
-            faces = near[i, cand]
-            face = int(faces[np.argmax(mesh.face_centers[faces] @ dirs[i])])
-        else:
-            face = face_index(mesh, lats[i], lons[i])
-        vertices = mesh.faces[face]
-        a, b, c = mesh.vertices[vertices]
-        normal = np.cross(b - a, c - a)
-        point = dirs[i] * float((normal @ a) / (normal @ dirs[i]))
-        ab, ac, off = b - a, c - a, point - a
-        d11, d12, d22 = float(ab @ ab), float(ab @ ac), float(ac @ ac)
-        o1, o2 = float(off @ ab), float(off @ ac)
-        den = d11 * d22 - d12 ** 2
-        wb = (d22 * o1 - d12 * o2) / den
-        wc = (d11 * o2 - d12 * o1) / den
-        weights = np.clip(np.asarray((1.0 - wb - wc, wb, wc)), 0.0, None)
-        out.append((vertices.copy(), 0, weights / weights.sum()))
-    return out
+    # the face: the one passing candidate; among several (a point on a shared
+    # edge or vertex) the one whose centre is nearest, as the per-point search
+    # picks it; without a candidate the full per-point search
+    counts = inside.sum(axis=1)
+    face = np.empty(len(dirs), dtype=np.int64)
+    one = counts == 1
+    face[one] = near[one, np.argmax(inside[one], axis=1)]
+    for i in np.flatnonzero(counts > 1):
+        faces = near[i, np.flatnonzero(inside[i])]
+        face[i] = int(faces[np.argmax(mesh.face_centers[faces] @ dirs[i])])
+    for i in np.flatnonzero(counts == 0):
+        face[i] = face_index(mesh, lats[i], lons[i])
+    # barycentric weights of every point at once; the stacked 3-vector dot
+    # products round as the per-point ones do, so the weights are bitwise
+    # those of barycentric_distribution
+    vertices = np.asarray(mesh.faces)[face]
+    corners = np.asarray(mesh.vertices)[vertices]
+    a, b, c = corners[:, 0], corners[:, 1], corners[:, 2]
+
+    def dot(x, y):
+        return np.matmul(x[:, None, :], y[:, :, None])[:, 0, 0]
+
+    ab, ac = b - a, c - a
+    normal = np.cross(ab, ac)
+    point = dirs * (dot(normal, a) / dot(normal, dirs))[:, None]
+    off = point - a
+    d11, d12, d22 = dot(ab, ab), dot(ab, ac), dot(ac, ac)
+    o1, o2 = dot(off, ab), dot(off, ac)
+    # Python's float power (C pow), as the per-point formula squares d12;
+    # an array ** 2 multiplies instead and can differ in the last bit
+    den = d11 * d22 - np.asarray([x ** 2 for x in d12.tolist()])
+    wb = (d22 * o1 - d12 * o2) / den
+    wc = (d11 * o2 - d12 * o1) / den
+    weights = np.clip(np.stack((1.0 - wb - wc, wb, wc), axis=1), 0.0, None)
+    weights = weights / weights.sum(axis=1, keepdims=True)
+    return [(vertices[i].copy(), 0, weights[i]) for i in range(len(dirs))]
 
 
 def _receiver_table(mesh, altitudes, lat: float, lon: float):

@@ -353,6 +353,34 @@ def test_trace_query_rings_and_pair_weights():
     assert tq.in_range(q, s5).all()
 
 
+def test_batched_surface_tables_equal_the_per_point_distribution_bitwise():
+    """The trace-query service's batched face search and barycentric weights
+    equal sources.barycentric_distribution point by point, bit for bit: at
+    random points and at the ties of the face search (mesh vertices, edge
+    midpoints, face centres, the poles and the antimeridian)."""
+    from ionosphere_fdtd_amd import trace_queries as tq
+    from ionosphere_fdtd_amd.sources import barycentric_distribution
+
+    mesh = g.build_geodesic_mesh(5)
+    rng = np.random.default_rng(17)
+    lat = np.degrees(np.arcsin(rng.uniform(-1.0, 1.0, 2000)))
+    lon = rng.uniform(-180.0, 180.0, 2000)
+    v = np.asarray(mesh.vertices)
+    e = np.asarray(mesh.edges)
+    mids = v[e[:200, 0]] + v[e[:200, 1]]
+    special = np.concatenate([v[:200], mids / np.linalg.norm(mids, axis=1, keepdims=True),
+                              np.asarray(mesh.face_centers)[:200]])
+    lat = np.concatenate([lat, np.degrees(np.arcsin(np.clip(special[:, 2], -1.0, 1.0))),
+                          [90.0, -90.0, 0.0]])
+    lon = np.concatenate([lon, np.degrees(np.arctan2(special[:, 1], special[:, 0])),
+                          [0.0, 0.0, 180.0]])
+    tables = tq.surface_tables(mesh, lat, lon)
+    for i, (la, lo) in enumerate(zip(lat, lon)):
+        verts, _, weights = barycentric_distribution(mesh, [0.0, 1.0], la, lo, 0.0)
+        assert np.array_equal(tables[i][0], verts), i
+        assert tables[i][2].tobytes() == weights.tobytes(), i
+
+
 def test_committee_sums_pair_traces_per_query_in_pair_order():
     """The committee's trace of a query is the sum of its pairs' weighted
     traces, added in pair order (the np.add.at restatement, bit for bit); the
