@@ -68,6 +68,15 @@ __device__ __forceinline__ float sigmoid(float x) { return 1.0f / (1.0f + __expf
 // element with zero fill.
 // ---------------------------------------------------------------------------
 constexpr int kTK = 32, kLdsRow = kTK + 8;
+
+// LDS column of value k of tile row `row`: the four 8-value k groups of a row
+// are permuted by bits 3-4 of the row (an XOR swizzle).  The 16-byte group
+// reads and writes stay aligned, and the transposing stores of row-contiguous
+// operands (8 rows apart per lane, one k each) spread over four banks instead
+// of one (rows 8 apart are 160 words apart: the same bank).
+__device__ __forceinline__ int lds_col(int row, int k) {
+  return (((k >> 3) ^ ((row >> 3) & 3)) << 3) | (k & 7);
+}
 constexpr int kTM = 64, kTN = 64;  // tile of the split-K path (scratch sizing)
 enum : int { kOpGeneric = 0, kOpKContig = 1, kOpRowContig = 2 };
 
@@ -115,9 +124,10 @@ struct OperandTile {
         const uint32_t w[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
 #pragma unroll
         for (int e = 0; e < 8; ++e)
-          lds[(rr + e) * kLdsRow + kk] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+          lds[(rr + e) * kLdsRow + lds_col(rr + e, kk)] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
       } else {
-        *reinterpret_cast<uint4*>(&lds[(q >> 2) * kLdsRow + (q & 3) * 8]) = v[j];
+        const int row = q >> 2;
+        *reinterpret_cast<uint4*>(&lds[row * kLdsRow + lds_col(row, (q & 3) * 8)]) = v[j];
       }
     }
   }
@@ -162,13 +172,17 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(
     }
     bf16x8 a[FI], b[FJ];
 #pragma unroll
-    for (int i = 0; i < FI; ++i)
+    for (int i = 0; i < FI; ++i) {
+      const int row = wm + i * 16 + r16;
       a[i] = __builtin_bit_cast(
-          bf16x8, *reinterpret_cast<const uint4*>(&As[buf][(wm + i * 16 + r16) * kLdsRow + kq]));
+          bf16x8, *reinterpret_cast<const uint4*>(&As[buf][row * kLdsRow + lds_col(row, kq)]));
+    }
 #pragma unroll
-    for (int j = 0; j < FJ; ++j)
+    for (int j = 0; j < FJ; ++j) {
+      const int row = wn + j * 16 + r16;
       b[j] = __builtin_bit_cast(
-          bf16x8, *reinterpret_cast<const uint4*>(&Bt[buf][(wn + j * 16 + r16) * kLdsRow + kq]));
+          bf16x8, *reinterpret_cast<const uint4*>(&Bt[buf][row * kLdsRow + lds_col(row, kq)]));
+    }
 #pragma unroll
     for (int i = 0; i < FI; ++i)
 #pragma unroll
