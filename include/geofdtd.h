@@ -335,7 +335,8 @@ int geo_ensemble_create(GeoPlan* const* plans, int32_t n_plans, GeoEnsemble** ou
 /* Column-block shape: strips_per_thread 1, 2 or 4 (16-byte strips a thread
  * updates per row); block_order bit 0: 0 run-major, 1 runs interleaved;
  * bit 1 (strips_per_thread 1 only): every run takes the flat thread mapping
- * (GEO_MAPPING_FLAT), where otherwise each run keeps its plan's mapping.
+ * (GEO_MAPPING_FLAT) that its flat indices allow (rows x strips < 2^31 per
+ * kernel), where otherwise each run keeps its plan's mapping.
  * All shapes give the same bits. */
 int geo_ensemble_configure(GeoEnsemble* ens, int32_t strips_per_thread, int32_t block_order);
 /* Autotuner: time every shape over `steps` source-free steps, keep the

@@ -152,12 +152,16 @@ int32_t idle_lanes(const geo::SegGeom& g) { return geo::kBlock - g.rows_per_bloc
 // strips, 6 and 16 idle lanes) E 0.311 -> 0.235 and H 0.299 -> 0.288 at
 // s7, E 1.235 -> 0.933 at s8; at 80 cells (4 and 16 idle) E 0.169 ->
 // 0.139; at 64 cells (1 and 0 idle) flat is slower: E 0.148 -> 0.150,
-// H 0.184 -> 0.189.  Flat indexing needs rows * strips < 2^31.
-bool use_flat(int32_t mapping, const geo::SegGeom& a, const geo::SegGeom& b) {
+// H 0.184 -> 0.189.  Flat indexing needs rows * strips < 2^31 (flat_fits).
+bool flat_fits(const geo::SegGeom& a, const geo::SegGeom& b) {
   const auto fits = [](const geo::SegGeom& g) {
     return (int64_t)g.n_rows * g.row_strips + geo::kBlock < (int64_t)INT32_MAX;
   };
-  if (!fits(a) || !fits(b) || mapping == GEO_MAPPING_ROWS) return false;
+  return fits(a) && fits(b);
+}
+
+bool use_flat(int32_t mapping, const geo::SegGeom& a, const geo::SegGeom& b) {
+  if (!flat_fits(a, b) || mapping == GEO_MAPPING_ROWS) return false;
   if (mapping == GEO_MAPPING_FLAT) return true;
   return idle_lanes(a) >= geo::kBlock / 16 || idle_lanes(b) >= geo::kBlock / 16;
 }
@@ -1788,13 +1792,13 @@ struct Ensemble final : EnsembleBase {
       host_ep[r].g_vertex = make_geom(p.V, p.pn, N, S);
       host_ep[r].g_edge = make_geom(p.E, p.pm, N, S);
       // one strip per thread: each run keeps the thread mapping its plan took,
-      // or every run goes flat (policy bit 1)
+      // or every run whose flat indices fit in 32 bits goes flat (policy bit 1)
       const bool all_flat = (policy & 2) != 0;
-      if (S == 1 && (p.h_flat || all_flat)) {
+      if (S == 1 && (p.h_flat || (all_flat && flat_fits(host_hp[r].g_edge, host_hp[r].g_face)))) {
         set_flat(&host_hp[r].g_edge);
         set_flat(&host_hp[r].g_face);
       }
-      if (S == 1 && (p.e_flat || all_flat)) {
+      if (S == 1 && (p.e_flat || (all_flat && flat_fits(host_ep[r].g_vertex, host_ep[r].g_edge)))) {
         set_flat(&host_ep[r].g_vertex);
         set_flat(&host_ep[r].g_edge);
       }
