@@ -120,11 +120,33 @@ struct OperandTile {
     for (int j = 0; j < ROWS / 64; ++j) {
       const int q = (int)threadIdx.x + 256 * j;
       if (MODE == kOpRowContig) {
+        // Lanes ROWS/8 apart hold the same 8 rows at k and k + 1 (k even).
+        // They swap halves: the even lane keeps rows rr..rr+3, the odd lane
+        // rows rr+4..rr+7, and each writes its 4 rows as (k, k + 1) pairs,
+        // 4 32-bit stores in place of 8 16-bit ones.
         const int kk = q / (ROWS / 8), rr = (q % (ROWS / 8)) * 8;
-        const uint32_t w[4] = {v[j].x, v[j].y, v[j].z, v[j].w};
+        const bool odd = (kk & 1) != 0;
+        const uint32_t keep0 = odd ? v[j].z : v[j].x, keep1 = odd ? v[j].w : v[j].y;
+        const uint32_t send0 = odd ? v[j].x : v[j].z, send1 = odd ? v[j].y : v[j].w;
+        uint32_t got0, got1;
+        if constexpr (ROWS / 8 == 8) {
+          // partner lane ^ 8: a rotation by 8 within each row of 16 lanes
+          // (DPP row_ror:8), with no trip through the LDS crossbar
+          got0 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send0, 0x128, 0xf, 0xf, false);
+          got1 = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send1, 0x128, 0xf, 0xf, false);
+        } else {
+          got0 = (uint32_t)__shfl_xor((int)send0, ROWS / 8);
+          got1 = (uint32_t)__shfl_xor((int)send1, ROWS / 8);
+        }
+        const uint32_t lo[2] = {odd ? got0 : keep0, odd ? got1 : keep1};  // values at k
+        const uint32_t hi[2] = {odd ? keep0 : got0, odd ? keep1 : got1};  // values at k + 1
+        const int r0 = rr + (odd ? 4 : 0), k_even = kk & ~1;
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
-          lds[(rr + e) * kLdsRow + lds_col(rr + e, kk)] = (uint16_t)(w[e >> 1] >> (16 * (e & 1)));
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t a = lo[e >> 1], b = hi[e >> 1];
+          const uint32_t pair = (e & 1) ? (a >> 16) | (b & 0xffff0000u) : (a & 0xffffu) | (b << 16);
+          *reinterpret_cast<uint32_t*>(&lds[(r0 + e) * kLdsRow + lds_col(r0 + e, k_even)]) = pair;
+        }
       } else {
         const int row = q >> 2;
         *reinterpret_cast<uint4*>(&lds[row * kLdsRow + lds_col(row, (q & 3) * 8)]) = v[j];
