@@ -25,7 +25,10 @@ extern "C" {
  * elements: A(m,k) = A[m*a_rs + k*a_cs], B(k,n) = B[k*b_rs + n*b_cs], so
  * transposed operands need no copy.  FP32 accumulation.  Outputs (row-major,
  * leading dimension ldc): c_f32 (written, or incremented when accumulate is
- * 1) and/or c_bf16; bias (N FP32 values) may be NULL.  Any M, N, K >= 1. */
+ * 1) and/or c_bf16; bias (N FP32 values) may be NULL.  Any M, N, K >= 1.
+ * A long K with few output tiles is split over blocks; the split-K partials
+ * (at most 8 MiB) are a stream-ordered allocation from a pool the library
+ * keeps per device, which holds freed memory for the next call. */
 int geo_gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs,
                   int64_t a_cs, const uint16_t* B, int64_t b_rs, int64_t b_cs, const float* bias,
                   float* c_f32, uint16_t* c_bf16, int64_t ldc, int32_t accumulate, void* stream);
@@ -58,7 +61,8 @@ int geo_silu_backward(int64_t n, const float* pre, const float* dout, float* dpr
 /* out[c] = sum over rows r < M of x[r*ld + c], c < N: fixed chunks of about
  * 256 rows summed per block, then the chunk sums in order, so the result is
  * the same bits on every call.  Takes a stream-ordered allocation for the
- * chunk sums. */
+ * chunk sums from the library's own pool, which keeps freed memory for the
+ * next call (as for geo_gemm_bf16's split-K partials). */
 int geo_column_sums(int64_t M, int64_t N, const float* x, int64_t ld, float* out, void* stream);
 
 /* Nearest-neighbour upsampling by `up` followed by the im2col of a
