@@ -618,9 +618,13 @@ def decompose(queries: np.ndarray, mesh):
     return (qi, us, vs, ws), covered
 
 
-def in_range(queries: np.ndarray, mesh=None) -> np.ndarray:
+def in_range(queries: np.ndarray, mesh=None, *, with_pairs: bool = False):
     """Physics in the ensemble box, receiver within MAX_DISTANCE_M of the
-    source, every weighted vertex pair within RINGS rings."""
+    source, every weighted vertex pair within RINGS rings.
+
+    With `with_pairs`, also returns decompose(queries[ok]) -- the pairs of
+    the in-range queries, indexed among them -- which the range test has
+    computed already (a query's pairs depend on that query alone)."""
     mesh = mesh if mesh is not None else build_geodesic_mesh(SUBDIVISION)
     q = np.asarray(queries, dtype=np.float64)
     ok = np.ones(q.shape[0], dtype=bool)
@@ -630,10 +634,14 @@ def in_range(queries: np.ndarray, mesh=None) -> np.ndarray:
     lo, hi = BOX["carrier_hz"]
     ok &= (q[:, 7] >= lo) & (q[:, 7] <= hi)
     ok &= np.all(np.isfinite(q), axis=1) & (np.abs(q[:, [3, 5]]) <= 90.0).all(axis=1)
+    empty = np.zeros(0, dtype=np.int64)
+    pairs = (empty, empty, empty, np.zeros(0))
     if ok.any():
-        _, covered = decompose(q[ok], mesh)
+        (qi, u, v, w), covered = decompose(q[ok], mesh)
         ok[np.flatnonzero(ok)] = covered
-    return ok
+        keep = covered[qi]
+        pairs = ((np.cumsum(covered) - 1)[qi[keep]], u[keep], v[keep], w[keep])
+    return (ok, pairs) if with_pairs else ok
 
 
 def normalise(traces: np.ndarray, floor: float = 1e-30):
@@ -687,16 +695,20 @@ class SurrogateModel:
     amp_std: float
     amp_coef: np.ndarray | None = None
 
-    def log_amplitude(self, features: np.ndarray, amp_n: np.ndarray) -> np.ndarray:
+    def log_amplitude(self, features: np.ndarray, amp_n: np.ndarray,
+                      basis: np.ndarray | None = None) -> np.ndarray:
+        """`basis`: amplitude_basis(features) when the caller holds it already."""
         amp = np.asarray(amp_n, dtype=np.float64) * self.amp_std + self.amp_mean
         if self.amp_coef is not None:
-            amp = amp + amplitude_basis(features) @ self.amp_coef
+            amp = amp + (amplitude_basis(features) if basis is None else basis) @ self.amp_coef
         return amp
 
-    def predict_pairs(self, features: np.ndarray) -> np.ndarray:
+    def predict_pairs(self, features: np.ndarray, basis: np.ndarray | None = None) -> np.ndarray:
         shape, amp_n = self.net.predict(features)
-        amp = self.log_amplitude(features, amp_n)
-        return shape.astype(np.float64) * np.exp(amp)[:, None]
+        amp = self.log_amplitude(features, amp_n, basis)
+        g = shape.astype(np.float64)
+        g *= np.exp(amp)[:, None]
+        return g
 
     def save(self, path) -> None:
         params = self.net.host_parameters()
@@ -728,14 +740,18 @@ class SurrogateCommittee:
     def load(cls, paths):
         return cls([SurrogateModel.load(p) for p in paths])
 
-    def predict(self, queries: np.ndarray, mesh) -> tuple[np.ndarray, np.ndarray]:
+    def predict(self, queries: np.ndarray, mesh, pairs=None) -> tuple[np.ndarray, np.ndarray]:
         """(mean trace (n, SAMPLES), member disagreement (n,)) for in-range queries:
         the disagreement is the largest relative L2 distance of a member's
-        trace from the committee mean."""
+        trace from the committee mean.  `pairs`: decompose(queries)[0] when
+        the caller holds it already (in_range(..., with_pairs=True))."""
         q = np.asarray(queries, dtype=np.float64)
-        (qi, u, v, w), _ = decompose(q, mesh)
+        qi, u, v, w = decompose(q, mesh)[0] if pairs is None else pairs
         dts = time_steps(q[:, 0], q[:, 1], mesh)
         feats = pair_features(q[qi][:, [0, 1, 2, 7]], dts[qi], u, v, mesh_rings(mesh))
+        # the members share the features, so the amplitude baseline's basis too
+        basis = amplitude_basis(feats) if any(m.amp_coef is not None for m in self.members) \
+            else None
         # Each query's trace is the sum of its pairs' weighted traces.  The
         # sparse (queries x pairs) product adds every query's pairs in pair
         # order, as np.add.at does (bit for bit), at a third of its cost
@@ -746,7 +762,8 @@ class SurrogateCommittee:
                              shape=(q.shape[0], len(qi)))
         per_member = []
         for m in self.members:
-            g = m.predict_pairs(feats) * w[:, None]
+            g = m.predict_pairs(feats, basis)
+            g *= w[:, None]
             per_member.append(np.asarray(scatter @ g))
         mean = np.mean(per_member, axis=0)
         spread = np.max([trace_rel_l2(t, mean) for t in per_member], axis=0)
@@ -818,7 +835,7 @@ class TraceQueryService:
         t0 = time.perf_counter()
         out = np.zeros((q.shape[0], SAMPLES))
         served = np.full(q.shape[0], "fdtd", dtype=object)
-        ok = in_range(q, self.mesh)
+        ok, pairs = in_range(q, self.mesh, with_pairs=True)
         report = {"queries": int(q.shape[0]), "in_range": int(ok.sum())}
         fdtd_rows = np.flatnonzero(~ok)
         rows = np.zeros(0, dtype=np.int64)
@@ -827,7 +844,7 @@ class TraceQueryService:
         if ok.any():
             rows = np.flatnonzero(ok)
             t1 = time.perf_counter()
-            pred, spread = self.model.predict(q[rows], self.mesh)
+            pred, spread = self.model.predict(q[rows], self.mesh, pairs)
             report["surrogate_seconds"] = time.perf_counter() - t1
             confident = spread <= self.disagreement
             report["uncertain"] = int(np.sum(~confident))

@@ -388,10 +388,12 @@ def test_committee_sums_pair_traces_per_query_in_pair_order():
     from ionosphere_fdtd_amd import trace_queries as tq
 
     class Member:  # a deterministic stand-in for a trained network
+        amp_coef = None  # no host amplitude baseline
+
         def __init__(self, phase):
             self.phase = phase
 
-        def predict_pairs(self, feats):
+        def predict_pairs(self, feats, basis=None):
             k = np.arange(tq.SAMPLES)
             return np.sin(np.outer(feats @ np.linspace(0.3, 1.1, feats.shape[1]), k / 97.0)
                           + self.phase) * np.exp(feats[:, :1])
@@ -413,3 +415,24 @@ def test_committee_sums_pair_traces_per_query_in_pair_order():
     np.testing.assert_array_equal(
         spread, np.max([tq.trace_rel_l2(t, want) for t in traces], axis=0))
     assert np.all(spread > 0)
+
+
+def test_range_test_hands_over_the_pairs_of_the_in_range_queries():
+    """in_range(..., with_pairs=True) returns decompose() of exactly the
+    in-range queries (indexed among them), so the service decomposes once;
+    queries outside the physics box or too far from their source drop out."""
+    from ionosphere_fdtd_amd import trace_queries as tq
+
+    s5 = g.build_geodesic_mesh(tq.SUBDIVISION)
+    q = tq.random_queries(9, 2, seed=21)
+    q[3, 0] = tq.BOX["h_prime_km"][1] + 5.0  # physics outside the box
+    q[10, 5], q[10, 6] = tq.destination_point(q[10, 3], q[10, 4], 0.0, 400e3, tq.EARTH_RADIUS_M)
+    ok, pairs = tq.in_range(q, s5, with_pairs=True)
+    assert np.array_equal(ok, tq.in_range(q, s5))
+    assert not ok[3] and not ok[10] and ok.sum() == len(q) - 2
+    want, covered = tq.decompose(q[ok], s5)
+    assert covered.all()
+    for got, exp in zip(pairs, want):
+        assert got.dtype == exp.dtype and got.tobytes() == exp.tobytes()
+    none_ok, none_pairs = tq.in_range(q[[3]], s5, with_pairs=True)
+    assert not none_ok.any() and all(len(a) == 0 for a in none_pairs)
