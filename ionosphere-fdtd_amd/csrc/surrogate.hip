@@ -159,14 +159,21 @@ struct OperandTile {
 // more than one split, writes its raw partial product to partial[z][m][n];
 // gemm_reduce_kernel then adds the partials in split order (deterministic).
 template <int TM, int TN, int AMODE, int BMODE>
-__global__ void __launch_bounds__(256) gemm_bf16_kernel(
+__global__ void __launch_bounds__(256)
+__attribute__((amdgpu_waves_per_eu(TM == 64 ? 5 : 1))) gemm_bf16_kernel(
     int64_t M, int64_t N, int64_t K, const uint16_t* __restrict__ A, int64_t a_rs, int64_t a_cs,
     const uint16_t* __restrict__ B, int64_t b_rs, int64_t b_cs, const float* __restrict__ bias,
     float* __restrict__ c32, uint16_t* __restrict__ c16, int64_t ldc, int32_t accumulate,
     int64_t k_chunk, float* __restrict__ partial) {
   constexpr int FI = TM / 32, FJ = TN / 32;  // 16 x 16 tiles per wave along m, n
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][TM * kLdsRow];
-  __shared__ __attribute__((aligned(16))) uint16_t Bt[2][TN * kLdsRow];
+  // the operand buffers, and after the K loop the output staging (64 rows of
+  // TN + 4 FP32 values at a time)
+  constexpr int kStageRow = TN + 4;
+  constexpr int kOperandBytes = 2 * (TM + TN) * kLdsRow * 2, kStageBytes = 64 * kStageRow * 4;
+  __shared__ __attribute__((aligned(16)))
+      uint16_t smem[(kOperandBytes > kStageBytes ? kOperandBytes : kStageBytes) / 2];
+  auto As = reinterpret_cast<uint16_t(*)[TM * kLdsRow]>(smem);
+  auto Bt = reinterpret_cast<uint16_t(*)[TN * kLdsRow]>(smem + 2 * TM * kLdsRow);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   const int64_t m0 = (int64_t)blockIdx.y * TM, n0 = (int64_t)blockIdx.x * TN;
   const int wm = (wave >> 1) * (TM / 2), wn = (wave & 1) * (TN / 2);
@@ -216,6 +223,79 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(
     }
     __syncthreads();
     buf ^= 1;
+  }
+  // Output rows as 16-byte vectors where every output allows them: the
+  // accumulators (4 rows x 16 columns per instruction, 64-byte row pieces)
+  // go through LDS, 64 tile rows at a time, and each thread then writes 4
+  // consecutive columns of one row.  Same values, same operations per value.
+  const auto al = [](const void* x, uintptr_t a) { return (reinterpret_cast<uintptr_t>(x) & (a - 1)) == 0; };
+  const bool vec_out = partial != nullptr
+                           ? (N % 4 == 0 && al(partial, 16))
+                           : (ldc % 4 == 0 && al(c32, 16) && al(c16, 8) && al(bias, 16));
+  if (vec_out) {
+    float* stage = reinterpret_cast<float*>(smem);
+    constexpr int kVecPerRow = TN / 4, kRowsPerIter = 256 / kVecPerRow;
+#pragma unroll
+    for (int pass = 0; pass < TM / 64; ++pass) {
+      if (wm / 64 == pass) {  // the waves whose quadrant holds these 64 rows
+#pragma unroll
+        for (int i = 0; i < FI; ++i)
+#pragma unroll
+          for (int j = 0; j < FJ; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+              stage[(wm - pass * 64 + i * 16 + (lane >> 4) * 4 + r) * kStageRow + wn + j * 16 + r16] =
+                  acc[i][j][r];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int it = 0; it < 64 / kRowsPerIter; ++it) {
+        const int lr = it * kRowsPerIter + t / kVecPerRow, c = (t % kVecPerRow) * 4;
+        const int64_t m = m0 + pass * 64 + lr, n = n0 + c;
+        if (m >= M || n >= N) continue;
+        const float4 s4 = *reinterpret_cast<const float4*>(&stage[lr * kStageRow + c]);
+        float v[4] = {s4.x, s4.y, s4.z, s4.w};
+        if (n + 4 <= N) {
+          if (partial != nullptr) {
+            *reinterpret_cast<float4*>(partial + ((int64_t)blockIdx.z * M + m) * N + n) = s4;
+            continue;
+          }
+          if (bias != nullptr) {
+            const float4 b4 = *reinterpret_cast<const float4*>(bias + n);
+            v[0] += b4.x; v[1] += b4.y; v[2] += b4.z; v[3] += b4.w;
+          }
+          if (c32 != nullptr) {
+            float4* p = reinterpret_cast<float4*>(c32 + m * ldc + n);
+            float4 o = make_float4(v[0], v[1], v[2], v[3]);
+            if (accumulate) {
+              const float4 q = *p;
+              o = make_float4(q.x + v[0], q.y + v[1], q.z + v[2], q.w + v[3]);
+            }
+            *p = o;
+          }
+          if (c16 != nullptr)
+            *reinterpret_cast<uint2*>(c16 + m * ldc + n) =
+                make_uint2((uint32_t)f2bf(v[0]) | ((uint32_t)f2bf(v[1]) << 16),
+                           (uint32_t)f2bf(v[2]) | ((uint32_t)f2bf(v[3]) << 16));
+        } else {  // the last columns of the matrix
+          for (int e = 0; e < 4 && n + e < N; ++e) {
+            float x = v[e];
+            if (partial != nullptr) {
+              partial[((int64_t)blockIdx.z * M + m) * N + n + e] = x;
+              continue;
+            }
+            if (bias != nullptr) x += bias[n + e];
+            if (c32 != nullptr) {
+              float* p = c32 + m * ldc + n + e;
+              *p = accumulate ? *p + x : x;
+            }
+            if (c16 != nullptr) c16[m * ldc + n + e] = f2bf(x);
+          }
+        }
+      }
+      if (pass + 1 < TM / 64) __syncthreads();  // the staging rows are rewritten
+    }
+    return;
   }
 #pragma unroll
   for (int i = 0; i < FI; ++i)
