@@ -638,9 +638,9 @@ int gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs, 
   splits = (K + k_chunk - 1) / k_chunk;
   const int am = operand_mode(A, a_rs, a_cs), bm = operand_mode(B, b_cs, b_rs);
   if (splits == 1) {
-    // 128 x 128 block tiles where both sides fill them and K takes more than
-    // two steps, 64 x 64 otherwise
-    if (M >= 128 && N >= 128 && K > 2 * kTK) {
+    // 128 x 128 block tiles where both sides fill them, K takes more than two
+    // steps and the tiles still give every CU a block; 64 x 64 otherwise
+    if (M >= 128 && N >= 128 && K > 2 * kTK && ((M + 127) / 128) * ((N + 127) / 128) >= 256) {
       const int64_t gy2 = (M + 127) / 128, gx2 = (N + 127) / 128;
       launch_gemm<128, 128>(am, bm, dim3((uint32_t)gx2, (uint32_t)gy2, 1), s, M, N, K, A, a_rs,
                             a_cs, B, b_rs, b_cs, bias, c_f32, c_bf16, ldc, accumulate, K,

@@ -68,12 +68,14 @@ def test_gemm_bf16_matches_fp64_product_of_bf16_operands(M, N, K, ta, tb):
     assert np.all(np.abs(got16 - want) <= 2.0 ** -8 * np.abs(want) + 2e-5 * scale + 1e-5)
 
 
-@pytest.mark.parametrize("M,N,K", [(200, 136, 77), (129, 300, 96), (1000, 130, 40)])
+@pytest.mark.parametrize("M,N,K", [(2056, 2056, 77), (200, 136, 77), (129, 300, 96),
+                                   (1000, 130, 40)])
 def test_gemm_tile_paths_and_operand_layouts_give_the_same_bits(M, N, K):
     """Every operand layout (K contiguous, row contiguous, a stride of 2 that
-    takes the element-wise loads) and both block tiles: products with M, N >=
-    128 and K > 64 run on 128 x 128 tiles, the same rows in chunks of 64 on
-    64 x 64 tiles (as every product with K <= 64); each output tile
+    takes the element-wise loads) and both block tiles: a product with M, N
+    >= 128, K > 64 and at least 256 tiles of 128 x 128 runs on them (2056 x
+    2056: 289 tiles, the last row and column ragged), the same rows in chunks
+    of 64 on 64 x 64 tiles, as every smaller product; each output tile
     accumulates the same K slices in the same order, so all must agree bit
     for bit, and with the FP64 product of the BF16 operands to FP32
     accumulation error.  Outputs with a row stride that is a multiple of 4
