@@ -229,7 +229,9 @@ class TraceSurrogate:
     def backward(self, d_shape, d_amp) -> dict:
         """Gradients (FP32, in self.grads) of the last forward pass."""
         cfg, P16, G = self.cfg, self.params16, self.grads
-        cache = self._cache
+        cache = getattr(self, "_cache", None)
+        if cache is None:
+            raise RuntimeError("backward needs a forward pass first (predict keeps no activations)")
         B = cache.batch
         ds = d_shape.contiguous().view(-1, 1)
         ds16 = self._empty(tuple(ds.shape), bf16=True)
@@ -314,16 +316,24 @@ class TraceSurrogate:
         self.adam_step(lr)
         return loss
 
-    def predict(self, features, batch: int = 4096) -> tuple[np.ndarray, np.ndarray]:
-        """(shape (n, L), log-amp (n,)) as host arrays."""
+    def predict(self, features, batch: int = 8192) -> tuple[np.ndarray, np.ndarray]:
+        """(shape (n, L), log-amp (n,)) as host arrays.  Batches of up to
+        `batch` rows (a row's result does not depend on its batch: every
+        GEMM tile gives the same bits) are gathered on the device and copied
+        to the host once.  The decoder's last GEMM has batch x length rows,
+        at most 65535 row tiles of 64, which caps the batch."""
         torch = self.torch
+        batch = max(1, min(batch, 65535 * 64 // self.cfg.length))
         x = torch.as_tensor(np.asarray(features, dtype=np.float32), device=self.device)
-        shapes, amps = [], []
-        for i in range(0, x.shape[0], batch):
+        n = x.shape[0]
+        shapes = torch.empty((n, self.cfg.length), dtype=torch.float32, device=self.device)
+        amps = torch.empty(n, dtype=torch.float32, device=self.device)
+        for i in range(0, n, batch):
             s, a = self.forward(x[i:i + batch])
-            shapes.append(s.cpu().numpy().copy())
-            amps.append(a.cpu().numpy().copy())
-        return np.concatenate(shapes), np.concatenate(amps)
+            shapes[i:i + s.shape[0]].copy_(s)
+            amps[i:i + s.shape[0]].copy_(a)
+        self._cache = None  # the activations of the last batch are not kept
+        return shapes.cpu().numpy(), amps.cpu().numpy()
 
     def host_parameters(self) -> dict[str, np.ndarray]:
         return {k: v.cpu().numpy().copy() for k, v in self.params.items()}
