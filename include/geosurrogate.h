@@ -25,7 +25,9 @@ extern "C" {
  * elements: A(m,k) = A[m*a_rs + k*a_cs], B(k,n) = B[k*b_rs + n*b_cs], so
  * transposed operands need no copy.  FP32 accumulation.  Outputs (row-major,
  * leading dimension ldc): c_f32 (written, or incremented when accumulate is
- * 1) and/or c_bf16; bias (N FP32 values) may be NULL.  Any M, N, K >= 1.
+ * 1) and/or c_bf16; bias (N FP32 values) may be NULL.  Any M, N, K >= 1
+ * (a product of more than 32767 x 128 rows runs as row chunks of that many,
+ * with the same kernels and bits).
  * A long K with few output tiles is split over blocks; the split-K partials
  * (at most 8 MiB) are a stream-ordered allocation from a pool the library
  * keeps per device, which holds freed memory for the next call. */

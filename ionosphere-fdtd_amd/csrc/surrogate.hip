@@ -610,25 +610,45 @@ void launch_gemm(int am, int bm, dim3 grid, hipStream_t s, int64_t M, int64_t N,
 #undef GEMM_CASE
 }
 
+// Rows per launch of a product taller than 65535 row tiles of 64 (grid.y):
+// a multiple of 128 and of 8, so every chunk keeps the operand and output
+// alignment of the whole product.
+constexpr int64_t kGemmRowChunk = 32767LL * 128;
+
 // C = A B (+ bias).  `scratch_partial` NULL: split-K partials come from the
 // library's stream-ordered pool (partial_pool); otherwise they go to that buffer of
-// kGemmScratchFloats values (device address of `scratch`).
+// kGemmScratchFloats values (device address of `scratch`).  `allow_split`
+// false keeps one K pass per tile (the row chunks of a tall product).
 int gemm_bf16(int64_t M, int64_t N, int64_t K, const uint16_t* A, int64_t a_rs, int64_t a_cs,
               const uint16_t* B, int64_t b_rs, int64_t b_cs, const float* bias, float* c_f32,
               uint16_t* c_bf16, int64_t ldc, int32_t accumulate, hipStream_t s,
-              float* scratch_partial) {
+              float* scratch_partial, bool allow_split = true) {
   if (M < 1 || N < 1 || K < 1 || !A || !B || (!c_f32 && !c_bf16) || ldc < N)
     return geo_internal_fail(GEO_ERR_INVALID, "gemm: bad sizes or null operands");
   if (accumulate && !c_f32)
     return geo_internal_fail(GEO_ERR_INVALID, "gemm: accumulation needs the FP32 output");
   const int64_t gy = (M + kTM - 1) / kTM, gx = (N + kTN - 1) / kTN;
-  if (gy > 65535 || gx > 2147483647)
-    return geo_internal_fail(GEO_ERR_INVALID, "gemm: more than 65535 row tiles");
+  if (gx >= (1LL << 24))  // grid.x times 256 threads stays below 2^32
+    return geo_internal_fail(GEO_ERR_INVALID, "gemm: more than 2^24 column tiles");
+  if (M > kGemmRowChunk) {
+    // Rows are independent: a tall product runs as row chunks with the same
+    // kernels, one K pass per tile in every chunk (the last one included, so
+    // that no row's sum is split where the others' are not).
+    for (int64_t m = 0; m < M; m += kGemmRowChunk) {
+      const int64_t rows = M - m < kGemmRowChunk ? M - m : kGemmRowChunk;
+      const int rc = gemm_bf16(rows, N, K, A + m * a_rs, a_rs, a_cs, B, b_rs, b_cs, bias,
+                               c_f32 ? c_f32 + m * ldc : nullptr,
+                               c_bf16 ? c_bf16 + m * ldc : nullptr, ldc, accumulate, s,
+                               scratch_partial, false);
+      if (rc != GEO_OK) return rc;
+    }
+    return GEO_OK;
+  }
   // Few output tiles over a long K (weight gradients: K = batch x length):
   // split K so that the launch fills the chip, at most 256 splits of >= 1024
   // and at most kMaxSplitTiles (split, tile) blocks.
   int64_t splits = 1;
-  if (gx * gy < kMaxSplitTiles && K >= 2048) {
+  if (allow_split && gx * gy < kMaxSplitTiles && K >= 2048) {
     splits = kMaxSplitTiles / (gx * gy);
     if (splits > K / 1024) splits = K / 1024;
     if (splits > 256) splits = 256;

@@ -316,14 +316,12 @@ class TraceSurrogate:
         self.adam_step(lr)
         return loss
 
-    def predict(self, features, batch: int = 8192) -> tuple[np.ndarray, np.ndarray]:
+    def predict(self, features, batch: int = 32768) -> tuple[np.ndarray, np.ndarray]:
         """(shape (n, L), log-amp (n,)) as host arrays.  Batches of up to
         `batch` rows (a row's result does not depend on its batch: every
-        GEMM tile gives the same bits) are gathered on the device and copied
-        to the host once.  The decoder's last GEMM has batch x length rows,
-        at most 65535 row tiles of 64, which caps the batch."""
+        GEMM tile and row chunk gives the same bits) are gathered on the
+        device and copied to the host once."""
         torch = self.torch
-        batch = max(1, min(batch, 65535 * 64 // self.cfg.length))
         x = torch.as_tensor(np.asarray(features, dtype=np.float32), device=self.device)
         n = x.shape[0]
         shapes = torch.empty((n, self.cfg.length), dtype=torch.float32, device=self.device)

@@ -68,6 +68,40 @@ def test_gemm_bf16_matches_fp64_product_of_bf16_operands(M, N, K, ta, tb):
     assert np.all(np.abs(got16 - want) <= 2.0 ** -8 * np.abs(want) + 2e-5 * scale + 1e-5)
 
 
+def test_gemm_taller_than_one_launch_runs_in_row_chunks():
+    """A product with more than 65535 row tiles of 64 (grid.y) runs as row
+    chunks of 32767 x 128 rows: rows on both sides of a chunk boundary and in
+    the short last chunk equal the same rows computed alone, bit for bit, and
+    the whole product matches the FP64 product of the BF16 operands."""
+    import torch
+
+    from ionosphere_fdtd_amd import _native
+    from ionosphere_fdtd_amd.surrogate import bf16_bits, bf16_to_f32
+
+    lib = _native.load_library()
+    stream = torch.cuda.current_stream().cuda_stream
+    chunk = 32767 * 128
+    M, N, K = chunk + 200, 24, 40
+    rng = np.random.default_rng(3)
+    a16 = bf16_bits(rng.standard_normal((M, K), dtype=np.float32))
+    b16 = bf16_bits(rng.standard_normal((K, N), dtype=np.float32))
+    A, B = _dev16(torch, a16), _dev16(torch, b16)
+    bias = torch.as_tensor(rng.standard_normal(N, dtype=np.float32), device="cuda")
+    full = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    _native.check(lib.geo_gemm_bf16(M, N, K, A.data_ptr(), K, 1, B.data_ptr(), N, 1,
+                                    bias.data_ptr(), full.data_ptr(), None, N, 0, stream), lib)
+    got = full.cpu().numpy()
+    for m0, rows in ((chunk - 100, 200), (chunk, 200), (0, 64)):
+        part = torch.empty((rows, N), dtype=torch.float32, device="cuda")
+        _native.check(lib.geo_gemm_bf16(rows, N, K, A.data_ptr() + 2 * m0 * K, K, 1, B.data_ptr(),
+                                        N, 1, bias.data_ptr(), part.data_ptr(), None, N, 0,
+                                        stream), lib)
+        assert part.cpu().numpy().tobytes() == got[m0:m0 + rows].tobytes(), m0
+    want = (bf16_to_f32(a16).astype(np.float64) @ bf16_to_f32(b16).astype(np.float64)
+            + bias.cpu().numpy())
+    np.testing.assert_allclose(got, want, rtol=0, atol=2e-5 * np.sqrt(K) + 1e-5)
+
+
 @pytest.mark.parametrize("M,N,K", [(2056, 2056, 77), (200, 136, 77), (129, 300, 96),
                                    (1000, 130, 40)])
 def test_gemm_tile_paths_and_operand_layouts_give_the_same_bits(M, N, K):
