@@ -520,8 +520,10 @@ def main() -> int:
     import numpy as np
     import torch
 
+    import __graft_entry__
     import scenarios as S
 
+    __graft_entry__.ensure_built()  # the in-tree HIP library, built once if missing
     if args.cpu_baseline_full:
         res = cpu_legs("vacuum_s7_r80", warmup=32, steps=32, repeats=3)
         for leg, v in res["legs"].items():

@@ -9,6 +9,13 @@ for p in (ROOT, ROOT / "tests", ROOT / "oracle"):
         sys.path.insert(0, str(p))
 
 
+def pytest_sessionstart(session):
+    # the tests load the in-tree HIP library; build it once if this tree has none
+    import __graft_entry__
+
+    __graft_entry__.ensure_built()
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a gfx950 GPU and the built HIP library")
     config.addinivalue_line("markers", "slow: long-running CPU test")
