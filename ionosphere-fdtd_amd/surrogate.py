@@ -316,11 +316,12 @@ class TraceSurrogate:
         self.adam_step(lr)
         return loss
 
-    def predict(self, features, batch: int = 32768) -> tuple[np.ndarray, np.ndarray]:
+    def predict(self, features, batch: int = 8191) -> tuple[np.ndarray, np.ndarray]:
         """(shape (n, L), log-amp (n,)) as host arrays.  Batches of up to
         `batch` rows (a row's result does not depend on its batch: every
         GEMM tile and row chunk gives the same bits) are gathered on the
-        device and copied to the host once."""
+        device and copied to the host once.  The default keeps the decoder's
+        last GEMM in one launch and the activations of a batch near 4 GB."""
         torch = self.torch
         x = torch.as_tensor(np.asarray(features, dtype=np.float32), device=self.device)
         n = x.shape[0]
